@@ -1,0 +1,1016 @@
+// api.hip -- C ABI of libbubbles_rp (include/bubbles_rp.h): set-up, the
+// step scheduler and data movement.  Host code; kernels live in
+// kernels_grid.hip and kernels_bubble.hip.
+//
+// The step is driven from the host: each pressure evaluation ends with one
+// small device-to-host read of the residual (and, after the first
+// evaluation, the bubble bounding box), which decides the Picard sub-loop
+// exactly as the reference's host loop does (bubble_CUDA.cu:240-276).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "brp_internal.h"
+
+using namespace brp;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Snapshot {
+    bool valid = false;
+    std::vector<void *> bufs;   // device copies, same order as the handle's buffer list
+    std::vector<size_t> sizes;
+    // scalar state
+    int pc, fcur, fn, fm, fz, rp_cur, pl_m, pl_n, pl_p, first_step;
+    Window W;
+    int64_t nstep;
+    double t, tx;
+};
+
+}  // namespace
+
+struct brp_handle {
+    brp_config cfg{};
+    std::string err;
+    int prec = BRP_PREC_FP64;
+    size_t fsz = 8;
+    GridParams g{};
+    BubbleParams bp{};
+    int N = 0, M = 0, pitch = 0, rows = 0;
+    size_t frame = 0;  // elements per frame
+    double T_inf = 0, P_inf = 0, rho_inf = 0, cs_inf = 0, dt = 0, dt0_rp = 0;
+    double fp_x = 0, fp_y = 0, omega = 0, ctrl_x = 0, ctrl_y = 0, ctrl_dist = 0;
+    std::vector<double> sig_mx, sig_my, sig_nx, sig_ny;
+
+    // device grid state
+    void *vx = nullptr, *vy = nullptr, *p0 = nullptr;
+    void *pb[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][x|y]
+    int pc = 0;                                                  // buffer holding p
+    int64_t *fgb[4] = {nullptr, nullptr, nullptr, nullptr};
+    int fcur = 0, fn = 1, fm = 2, fz = 3;
+    void *cxa = nullptr, *cxb = nullptr, *cya = nullptr, *cyb = nullptr;
+    void *ax = nullptr, *bx = nullptr, *ay = nullptr, *by = nullptr;
+    double *xu_d = nullptr, *rxp_d = nullptr;
+    double *el_amp_d = nullptr, *el_phase_d = nullptr;
+    int n_elem = 0;
+
+    // bubbles
+    BubbleArrays b{};
+    int nb = 0;
+    int rp_cur = 0, pl_m = 0, pl_n = 1, pl_p = 2;
+    std::vector<int> perm;  // perm[k] = caller index of sorted bubble k
+    std::vector<double> pending_x, pending_y, pending_R0, pending_nB;  // host-side set-up
+
+    StepScalars *sc_d = nullptr;
+    StepScalars *sc_h = nullptr;
+    Window W{1, 0, 1, 0};
+    int first_step = 1;
+    int64_t nstep = 0;
+    double t = 0.0, tx = 0.0;
+    bool prepared = false;
+
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev;
+    double phase_ms[5] = {0, 0, 0, 0, 0};
+
+    std::vector<void *> allocs;
+    std::vector<size_t> alloc_sizes;
+    size_t dev_bytes = 0;
+    Snapshot snaps[16];
+};
+
+namespace {
+
+int set_err(brp_handle *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h)
+        h->err = buf;
+    else
+        g_create_error = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_err(h, BRP_ERR_DEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                          \
+    } while (0)
+
+int dalloc(brp_handle *h, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess)
+        return set_err(h, BRP_ERR_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    e = hipMemsetAsync(*p, 0, bytes, h->stream);
+    if (e != hipSuccess)
+        return set_err(h, BRP_ERR_DEVICE, "hipMemset: %s", hipGetErrorString(e));
+    h->allocs.push_back(*p);
+    h->alloc_sizes.push_back(bytes);
+    h->dev_bytes += bytes;
+    return BRP_OK;
+}
+
+// water properties used by the grid (bubble_CUDA_kernel.cuh:153-169)
+double density_water() { return 1000.0; }
+double sound_speed_water() { return 1500.0; }
+
+// init_sigma (bubble_CUDA.cu:946-1205) for order 2: cell profiles mx, my
+// indexed by cell i (0..N+1) and node profiles nx, ny indexed by node + 1.
+void init_sigma(brp_handle *h) {
+    const brp_config &c = h->cfg;
+    const int N = c.grid_x, M = c.grid_y;
+    h->sig_mx.assign(N + 2, 0.0);
+    h->sig_my.assign(M + 2, 0.0);
+    h->sig_nx.assign(N + 3, 0.0);
+    h->sig_ny.assign(M + 3, 0.0);
+    const int npml = c.pml_grid_number;
+    const double order = c.pml_order;
+    auto smax = [&](double d) {
+        return -std::log(c.pml_sigma) * 0.5 * ((double)order + 1.0) / (d * (double)npml);
+    };
+    auto cell = [&](double s, int n) { return s * std::pow(((double)n - 0.5) / ((double)npml), order); };
+    auto node = [&](double s, int n) { return s * std::pow(((double)n) / ((double)npml), order); };
+    // ms = 0, me = 1, ns = -1, ne = 0 for order 2
+    if (c.pml_x0)
+        for (int i = 1; i <= std::min(N, npml); i++) h->sig_mx[i] = cell(smax(h->g.dx), npml - i + 1);
+    if (c.pml_x1 && N - npml + 1 <= N)
+        for (int i = std::max(1, N - npml + 1); i <= N; i++) h->sig_mx[i] = cell(smax(h->g.dx), i - N + npml);
+    if (c.pml_y0)
+        for (int j = 1; j <= std::min(M, npml); j++) h->sig_my[j] = cell(smax(h->g.dy), npml - j + 1);
+    if (c.pml_y1 && M - npml + 1 <= M)
+        for (int j = std::max(1, M - npml + 1); j <= M; j++) h->sig_my[j] = cell(smax(h->g.dy), j - M + npml);
+    if (c.pml_x0)
+        for (int i = 0; i <= std::min(N, npml - 1); i++) h->sig_nx[i + 1] = node(smax(h->g.dx), npml - i);
+    if (c.pml_x1 && N - npml + 1 <= N)
+        for (int i = std::max(0, N - npml + 1); i <= std::min(N, N + 2); i++)
+            h->sig_nx[i + 1] = node(smax(h->g.dx), i - N + npml);
+    if (c.pml_y0)
+        for (int j = 0; j <= std::min(M, npml - 1); j++) h->sig_ny[j + 1] = node(smax(h->g.dy), npml - j);
+    if (c.pml_y1 && M - npml + 1 <= M)
+        for (int j = std::max(0, M - npml + 1); j <= std::min(M, M + 2); j++)
+            h->sig_ny[j + 1] = node(smax(h->g.dy), j - M + npml);
+}
+
+template <typename F>
+int upload_coef(brp_handle *h, void **dst, const std::vector<double> &v) {
+    std::vector<F> tmp(v.size());
+    for (size_t k = 0; k < v.size(); k++) tmp[k] = (F)v[k];
+    int rc = dalloc(h, dst, sizeof(F) * v.size());
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpy(*dst, tmp.data(), sizeof(F) * v.size(), hipMemcpyHostToDevice));
+    return BRP_OK;
+}
+
+int upload_d(brp_handle *h, double **dst, const std::vector<double> &v) {
+    int rc = dalloc(h, (void **)dst, sizeof(double) * std::max<size_t>(v.size(), 1));
+    if (rc) return rc;
+    if (!v.empty())
+        HIPCHK(h, hipMemcpy(*dst, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+    return BRP_OK;
+}
+
+// Coefficient arrays of the velocity and pressure updates, computed in FP64
+// with the reference's operation order so the FP64 mode reproduces it:
+//   velocity s3 = ((c + c) * 0.5) * 0.5 * dt * sigma_n  (kernel.cuh:662-664)
+//   pressure s2 = (0.5 * dt * c) * sigma_m              (kernel.cuh:853-856)
+template <typename F>
+int init_coefficients(brp_handle *h) {
+    const double c = h->cs_inf, dt = h->dt;
+    auto mk = [&](const std::vector<double> &sig, bool vel, std::vector<double> &a,
+                  std::vector<double> &b) {
+        a.resize(sig.size());
+        b.resize(sig.size());
+        for (size_t k = 0; k < sig.size(); k++) {
+            double s;
+            if (vel) {
+                s = (c + c) * 0.5;
+                s = s * 0.5 * dt * sig[k];
+            } else {
+                s = (0.5 * dt * c) * sig[k];
+            }
+            a[k] = 1.0 - s;
+            b[k] = 1.0 + s;
+        }
+    };
+    std::vector<double> a, b;
+    int rc;
+    mk(h->sig_nx, true, a, b);
+    if ((rc = upload_coef<F>(h, &h->cxa, a)) || (rc = upload_coef<F>(h, &h->cxb, b))) return rc;
+    mk(h->sig_ny, true, a, b);
+    if ((rc = upload_coef<F>(h, &h->cya, a)) || (rc = upload_coef<F>(h, &h->cyb, b))) return rc;
+    // cell profiles are stored at index i + 1 like every 1-D array
+    std::vector<double> mx(h->N + 3, 0.0), my(h->M + 3, 0.0);
+    for (int i = 0; i <= h->N + 1; i++) mx[i + 1] = h->sig_mx[i];
+    for (int j = 0; j <= h->M + 1; j++) my[j + 1] = h->sig_my[j];
+    mk(mx, false, a, b);
+    if ((rc = upload_coef<F>(h, &h->ax, a)) || (rc = upload_coef<F>(h, &h->bx, b))) return rc;
+    mk(my, false, a, b);
+    if ((rc = upload_coef<F>(h, &h->ay, a)) || (rc = upload_coef<F>(h, &h->by, b))) return rc;
+    // init_grid_vector (bubble_CUDA.cu:690-713): xu at nodes, rxp at cells, index + 1
+    std::vector<double> xu(h->N + 3, 0.0), rxp(h->N + 3, 0.0);
+    for (int i = -1; i <= h->N + 1; i++) xu[i + 1] = ((double)i) * h->g.dx;
+    for (int i = 0; i <= h->N + 1; i++) rxp[i + 1] = 1.0 / (h->g.dx * ((double)i - 0.5));
+    if ((rc = upload_d(h, &h->xu_d, xu)) || (rc = upload_d(h, &h->rxp_d, rxp))) return rc;
+    return BRP_OK;
+}
+
+// Aperture of the focused source: row-0 cells with |fp - (xm, -dy/2)| <= f_dist.
+void find_aperture(brp_handle *h) {
+    h->g.ap_lo = 1;
+    h->g.ap_hi = 0;
+    const double ym = -0.5 * h->g.dy;
+    for (int i = 1; i <= h->N; i++) {
+        double xm = (i - 0.5) * h->g.dx;
+        double s1 = std::sqrt((h->ctrl_x - xm) * (h->ctrl_x - xm) + (h->ctrl_y - ym) * (h->ctrl_y - ym));
+        if (s1 <= h->ctrl_dist) {
+            if (h->g.ap_hi < h->g.ap_lo) h->g.ap_lo = i;
+            h->g.ap_hi = i;
+        }
+    }
+}
+
+inline int host_bin(double x, double r, double c) {
+    double v = std::floor(x * r + c);
+    return (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
+}
+
+int alloc_bubbles(brp_handle *h, int n) {
+    h->nb = n;
+    h->b.n = n;
+    const size_t bytes = sizeof(double) * (size_t)std::max(n, 1);
+    int rc;
+    double **arrs[] = {&h->b.x,      &h->b.y,      &h->b.R[0],   &h->b.R[1],   &h->b.Rn[0],
+                       &h->b.Rn[1],  &h->b.d1R[0], &h->b.d1R[1], &h->b.PG[0],  &h->b.PG[1],
+                       &h->b.bdt[0], &h->b.bdt[1], &h->b.re[0],  &h->b.re[1],  &h->b.R_t,
+                       &h->b.Q_B,    &h->b.PL[0],  &h->b.PL[1],  &h->b.PL[2],  &h->b.n_B,
+                       &h->b.R0,     &h->b.PG0,    &h->b.calpha, &h->b.vx,     &h->b.vy};
+    for (double **a : arrs)
+        if ((rc = dalloc(h, (void **)a, bytes))) return rc;
+    if ((rc = dalloc(h, (void **)&h->b.sub, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
+    return BRP_OK;
+}
+
+// bubble_input (bubble_CUDA.cu:900-943) for every bubble, in cell-sorted
+// order, with per-bubble R0 (extension E3).  Bubbles must not be re-placed
+// after brp_prepare.
+int place_bubbles(brp_handle *h, const std::vector<double> &x, const std::vector<double> &y,
+                  const std::vector<double> &R0v, const std::vector<double> *nBv, double fg_in) {
+    const int n = (int)x.size();
+    if (h->b.x) return set_err(h, BRP_ERR_STATE, "bubbles already placed");
+    int rc = alloc_bubbles(h, n);
+    if (rc) return rc;
+    // sort by (ibn.y, ibn.x) so scatter stencils of neighbouring lanes coincide
+    h->perm.resize(n);
+    std::iota(h->perm.begin(), h->perm.end(), 0);
+    std::vector<int64_t> key(n);
+    for (int k = 0; k < n; k++) {
+        int64_t kx = host_bin(x[k], h->g.rdx, 0.5), ky = host_bin(y[k], h->g.rdy, 0.5);
+        key[k] = (ky + 2000000000LL) * 4000000000LL + (kx + 2000000000LL);
+    }
+    std::stable_sort(h->perm.begin(), h->perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    const double Pi = std::acos(-1.0);
+    const brp_config &c = h->cfg;
+    std::vector<double> hx(n), hy(n), R0(n), PG0(n), ca(n), nB(n), dt0(n), zero(n, 0.0);
+    for (int k = 0; k < n; k++) {
+        const int q = h->perm[k];
+        hx[k] = x[q];
+        hy[k] = y[q];
+        R0[k] = R0v[q];
+        const double R03 = R0[k] * R0[k] * R0[k];
+        PG0[k] = c.bub_P0 + 2.0 * c.bub_sigma / R0[k];
+        ca[k] = c.bub_gamma * PG0[k] * R03 / (2.0 * (c.bub_gamma - 1.0) * c.bub_T0 * c.bub_K0);
+        if (nBv)
+            nB[k] = (*nBv)[q];
+        else if (c.tr_cylindrical)
+            nB[k] = fg_in * (hx[k] * h->g.dx * h->g.dy) / (4.0 / 3.0 * Pi * std::pow(R0[k], 3));
+        else
+            nB[k] = fg_in * (h->g.dx * h->g.dy) / (4.0 / 3.0 * Pi * std::pow(R0[k], 3));
+        dt0[k] = h->dt0_rp;
+    }
+    auto up = [&](double *d, const std::vector<double> &v) -> int {
+        if (n == 0) return BRP_OK;
+        HIPCHK(h, hipMemcpy(d, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        return BRP_OK;
+    };
+    if ((rc = up(h->b.x, hx)) || (rc = up(h->b.y, hy)) || (rc = up(h->b.R0, R0)) ||
+        (rc = up(h->b.PG0, PG0)) || (rc = up(h->b.calpha, ca)) || (rc = up(h->b.n_B, nB)) ||
+        (rc = up(h->b.R_t, R0)) || (rc = up(h->b.Q_B, zero)) || (rc = up(h->b.vx, zero)) ||
+        (rc = up(h->b.vy, zero)))
+        return rc;
+    for (int s = 0; s < 2; s++) {
+        if ((rc = up(h->b.R[s], R0)) || (rc = up(h->b.Rn[s], R0)) || (rc = up(h->b.d1R[s], zero)) ||
+            (rc = up(h->b.PG[s], PG0)) || (rc = up(h->b.bdt[s], dt0)) || (rc = up(h->b.re[s], zero)))
+            return rc;
+    }
+    for (int s = 0; s < 3; s++)
+        if ((rc = up(h->b.PL[s], zero))) return rc;
+    h->rp_cur = 0;
+    h->pl_m = 0;
+    h->pl_n = 1;
+    h->pl_p = 2;
+    return BRP_OK;
+}
+
+void window_union(Window &w, int i0, int i1, int j0, int j1) {
+    if (i0 > i1 || j0 > j1) return;
+    if (w.i0 > w.i1 || w.j0 > w.j1) {
+        w = Window{i0, i1, j0, j1};
+        return;
+    }
+    w.i0 = std::min(w.i0, i0);
+    w.i1 = std::max(w.i1, i1);
+    w.j0 = std::min(w.j0, j0);
+    w.j1 = std::max(w.j1, j1);
+}
+
+template <typename F>
+int read_frame(brp_handle *h, const void *src, std::vector<double> &out) {
+    std::vector<F> tmp(h->frame);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), src, sizeof(F) * h->frame, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    out.resize(h->frame);
+    for (size_t k = 0; k < h->frame; k++) out[k] = (double)tmp[k];
+    return BRP_OK;
+}
+
+int read_frame_fg(brp_handle *h, const int64_t *src, std::vector<double> &out) {
+    std::vector<int64_t> tmp(h->frame);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), src, sizeof(int64_t) * h->frame, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    out.resize(h->frame);
+    for (size_t k = 0; k < h->frame; k++) out[k] = (double)tmp[k] * kFgInvScale;
+    return BRP_OK;
+}
+
+void field_extent(const brp_handle *h, int field, int *nx, int *ny, int *i0, int *j0) {
+    *i0 = 0;
+    *j0 = 0;
+    *nx = h->N + 2;
+    *ny = h->M + 2;
+    if (field == BRP_FIELD_VX) {
+        *i0 = -1;
+        *nx = h->N + 3;
+    } else if (field == BRP_FIELD_VY) {
+        *j0 = -1;
+        *ny = h->M + 3;
+    }
+}
+
+// ---- the coupled step ----------------------------------------------------
+template <typename F>
+int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
+    const double thr = (double)1.0e-7f;  // bubble_CUDA.cu:243
+    const int max_pic = std::min(h->cfg.max_picard > 0 ? h->cfg.max_picard : 1000, kMaxEval - 1);
+    const bool bub = h->cfg.bub_enable && h->nb > 0;
+    hipStream_t s = h->stream;
+    const size_t hdr = offsetof(StepScalars, resid);
+    F *vx = (F *)h->vx, *vy = (F *)h->vy, *p0 = (F *)h->p0;
+    const F *cxa = (const F *)h->cxa, *cxb = (const F *)h->cxb, *cya = (const F *)h->cya,
+            *cyb = (const F *)h->cyb, *ax = (const F *)h->ax, *bx = (const F *)h->bx,
+            *ay = (const F *)h->ay, *by = (const F *)h->by;
+    for (int step = 0; step < k; step++) {
+        brp_step_stats st{};
+        // clock: compensated sum (bubble_CUDA.cu:208-212)
+        h->nstep++;
+        const double s1 = h->t;
+        h->t = h->t + h->dt + h->tx;
+        const double s2 = h->t - s1;
+        h->tx = h->dt + h->tx - s2;
+        // store_variables (kernel.cuh:1229-1262) as renamings: pn <- p,
+        // (f_gm, f_gn, f_g) <- (f_gn, f_g, prediction), PL history shift.
+        const int pn_i = h->pc, p_i = h->pc ^ 1;
+        const int old_cur = h->fcur, old_n = h->fn, old_m = h->fm;
+        {
+            const int m = h->pl_m, n = h->pl_n, p = h->pl_p;
+            h->pl_m = n;
+            h->pl_n = p;
+            h->pl_p = m;
+        }
+        launch_zero_scalars(s, h->sc_d, max_pic + 1);
+        HIPCHK(h, hipEventRecord(h->ev[0], s));
+        launch_velocity<F>(s, h->g, vx, vy, p0, h->fgb[old_cur], h->W, h->first_step, cxa, cxb,
+                           cya, cyb);
+        launch_velocity_ghosts<F>(s, h->g, vx, vy, h->t);
+        HIPCHK(h, hipEventRecord(h->ev[1], s));
+        if (bub) launch_motion<F>(s, h->g, h->bp, h->b, vx, vy, h->sc_d);
+        HIPCHK(h, hipEventRecord(h->ev[2], s));
+        launch_pressure_first<F>(s, h->g, vx, vy, (F *)h->pb[pn_i][0], (F *)h->pb[pn_i][1],
+                                 (F *)h->pb[p_i][0], (F *)h->pb[p_i][1], p0, h->fgb[old_cur],
+                                 h->fgb[old_n], h->fgb[old_m], h->W, ax, bx, ay, by, h->xu_d,
+                                 h->rxp_d, h->sc_d);
+        launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
+        HIPCHK(h, hipEventRecord(h->ev[3], s));
+        HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr + sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        HIPCHK(h, hipStreamSynchronize(s));
+        double r0;
+        std::memcpy(&r0, &h->sc_h->resid[0], 8);
+        double resimax = r0 / h->P_inf;
+        st.resimax0 = resimax;
+        if (bub) window_union(h->W, h->sc_h->bbox[0], h->sc_h->bbox[1], h->sc_h->bbox[2], h->sc_h->bbox[3]);
+        int status = BRP_OK;
+        int loop = 0;
+        int target = h->fz, prev = old_m;
+        double bub_ms = 0.0, pic_p_ms = 0.0;
+        if (bub) {
+            while (resimax > thr) {
+                if (loop >= max_pic) {
+                    status = BRP_ERR_PICARD_CAP;
+                    break;
+                }
+                loop++;
+                HIPCHK(h, hipEventRecord(h->ev[4], s));
+                launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
+                                      p0, h->fgb[target], h->sc_d, 1);
+                HIPCHK(h, hipEventRecord(h->ev[5], s));
+                launch_pressure_picard<F>(s, h->g, vx, vy, (F *)h->pb[pn_i][0], (F *)h->pb[pn_i][1],
+                                          (F *)h->pb[p_i][0], (F *)h->pb[p_i][1], p0,
+                                          h->fgb[target], h->fgb[old_cur], h->fgb[prev], h->W, ax,
+                                          bx, ay, by, h->xu_d, h->rxp_d, h->sc_d, loop);
+                launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
+                HIPCHK(h, hipEventRecord(h->ev[6], s));
+                HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr, hipMemcpyDeviceToHost, s));
+                HIPCHK(h, hipMemcpyAsync(&h->sc_h->resid[loop], &h->sc_d->resid[loop], 8,
+                                         hipMemcpyDeviceToHost, s));
+                HIPCHK(h, hipStreamSynchronize(s));
+                double r;
+                std::memcpy(&r, &h->sc_h->resid[loop], 8);
+                resimax = r / h->P_inf;
+                std::swap(target, prev);
+                float a = 0.f, c = 0.f;
+                (void)hipEventElapsedTime(&a, h->ev[4], h->ev[5]);
+                (void)hipEventElapsedTime(&c, h->ev[5], h->ev[6]);
+                bub_ms += a;
+                pic_p_ms += c;
+                if (h->sc_h->flags & 4) {
+                    status = BRP_ERR_SUBSTEP_CAP;
+                    break;
+                }
+            }
+        }
+        // rotate the void-fraction buffers: final iterate in `prev`, zero buffer in `target`
+        h->fcur = prev;
+        h->fz = target;
+        h->fn = old_cur;
+        h->fm = old_n;
+        if (loop > 0) h->rp_cur ^= 1;
+        if (bub && loop == 0) launch_copy_d(s, h->b.PL[h->pl_p], h->b.PL[h->pl_n], h->nb);
+        h->pc = p_i;
+        h->first_step = 0;
+        if (!bub) HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr, hipMemcpyDeviceToHost, s));
+        HIPCHK(h, hipStreamSynchronize(s));
+        {
+            float a = 0.f, b2 = 0.f, c = 0.f;
+            (void)hipEventElapsedTime(&a, h->ev[0], h->ev[1]);
+            (void)hipEventElapsedTime(&b2, h->ev[1], h->ev[2]);
+            (void)hipEventElapsedTime(&c, h->ev[2], h->ev[3]);
+            h->phase_ms[0] += a;
+            h->phase_ms[1] += b2;
+            h->phase_ms[2] += c + pic_p_ms;
+            h->phase_ms[3] += bub_ms;
+        }
+        const int flags = h->sc_h->flags;
+        if (status == BRP_OK && (flags & 3)) status = BRP_ERR_NONFINITE;
+        if (status == BRP_OK && (flags & 4)) status = BRP_ERR_SUBSTEP_CAP;
+        st.nstep = h->nstep;
+        st.time = h->t;
+        st.resimax = resimax;
+        st.n_pic = loop;
+        st.rp_max_substeps = h->sc_h->sub_max;
+        st.rp_mean_substeps = (loop > 0 && h->nb > 0)
+                                  ? (double)h->sc_h->sub_sum / ((double)h->nb * loop)
+                                  : 0.0;
+        st.status = status;
+        if (stats) stats[step] = st;
+        if (status != BRP_OK)
+            return set_err(h, status, "step %lld: %s (flags %d, Picard %d)", (long long)h->nstep,
+                           status == BRP_ERR_PICARD_CAP ? "Picard cap reached"
+                           : status == BRP_ERR_SUBSTEP_CAP ? "RP sub-step cap reached"
+                                                           : "non-finite value or void fraction out of range",
+                           flags, loop);
+    }
+    return BRP_OK;
+}
+
+template <typename F>
+int alloc_grid(brp_handle *h) {
+    int rc;
+    const size_t fb = sizeof(F) * h->frame;
+    if ((rc = dalloc(h, &h->vx, fb)) || (rc = dalloc(h, &h->vy, fb)) || (rc = dalloc(h, &h->p0, fb)))
+        return rc;
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 2; c++)
+            if ((rc = dalloc(h, &h->pb[a][c], fb))) return rc;
+    for (int a = 0; a < 4; a++)
+        if ((rc = dalloc(h, (void **)&h->fgb[a], sizeof(int64_t) * h->frame))) return rc;
+    return init_coefficients<F>(h);
+}
+
+}  // namespace
+
+extern "C" {
+
+void brp_config_default(brp_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->sim_order = 2;
+    cfg->sim_smooth_delta_band = 2;
+    cfg->pml_order = 2;
+    cfg->pml_sigma = 0.1;
+    cfg->sim_save_interval = 100;
+    cfg->tr_pid_start_step = 1;
+    cfg->tr_pid_iterate = 1;
+    cfg->precision = BRP_PREC_FP64;
+}
+
+const char *brp_last_error(const brp_handle *h) {
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int brp_create(const brp_config *cfg, brp_handle **out) {
+    if (!out) return set_err(nullptr, BRP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!cfg) return set_err(nullptr, BRP_ERR_INVALID, "cfg is NULL");
+    const brp_config &c = *cfg;
+    if (c.grid_x < 4 || c.grid_y < 4 || !(c.grid_lx > 0) || !(c.grid_ly > 0))
+        return set_err(nullptr, BRP_ERR_INVALID, "grid.x, grid.y >= 4 and grid.lx, grid.ly > 0 required");
+    if (c.grid_x > 65536 || c.grid_y > 65536)
+        return set_err(nullptr, BRP_ERR_INVALID, "grid.x and grid.y must be <= 65536");
+    if (c.sim_order != 2) return set_err(nullptr, BRP_ERR_INVALID, "simulation.order must be 2");
+    if (c.sim_smooth_delta_band < 1 || c.sim_smooth_delta_band > 8)
+        return set_err(nullptr, BRP_ERR_INVALID, "simulation.smooth_delta_band must be in [1, 8]");
+    if ((c.pml_x0 || c.pml_x1 || c.pml_y0 || c.pml_y1) &&
+        (c.pml_grid_number < 1 || !(c.pml_sigma > 0.0)))
+        return set_err(nullptr, BRP_ERR_INVALID, "pml.grid_number >= 1 and pml.sigma > 0 required with PML");
+    if (!(c.sim_cfl > 0)) return set_err(nullptr, BRP_ERR_INVALID, "simulation.cfl must be > 0");
+    if (c.precision != BRP_PREC_FP64 && c.precision != BRP_PREC_MIXED)
+        return set_err(nullptr, BRP_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
+    if (c.thermal) return set_err(nullptr, BRP_ERR_INVALID, "thermal solve is not available in this build");
+    if (c.reserved0) return set_err(nullptr, BRP_ERR_INVALID, "reserved0 must be 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(nullptr, BRP_ERR_NO_DEVICE, "no HIP device visible");
+
+    brp_handle *h = new brp_handle();
+    h->cfg = c;
+    h->prec = c.precision;
+    h->fsz = h->prec == BRP_PREC_FP64 ? 8 : 4;
+    h->N = c.grid_x;
+    h->M = c.grid_y;
+    h->pitch = ((h->N + 66 + 63) / 64) * 64;
+    h->rows = h->M + 3;
+    h->frame = (size_t)h->pitch * h->rows;
+    auto fail = [&](int rc) {
+        g_create_error = h->err;
+        brp_destroy(h);
+        return rc;
+    };
+    if (hipStreamCreate(&h->stream) != hipSuccess)
+        return fail(set_err(h, BRP_ERR_DEVICE, "hipStreamCreate failed"));
+    h->ev.resize(8);
+    for (auto &e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(set_err(h, BRP_ERR_DEVICE, "hipEventCreate failed"));
+
+    // init_grid_size (bubble_CUDA.cu:524-540)
+    GridParams &g = h->g;
+    g.N = h->N;
+    g.M = h->M;
+    g.pitch = h->pitch;
+    g.dx = (double)c.grid_lx / (double)c.grid_x;
+    g.dy = (double)c.grid_ly / (double)c.grid_y;
+    g.rdx = (double)1.0 / (double)g.dx;
+    g.rdy = (double)1.0 / (double)g.dy;
+    // init_transducer (bubble_CUDA.cu:543-558), E2 focus x
+    h->fp_x = 0.0;
+    h->fp_y = c.tr_focal_dist ? c.tr_focal_dist * 0.5 * std::sqrt(3.0) : c.grid_ly * 0.5;
+    if (c.ext_focus_x_set) h->fp_x = c.ext_focus_x;
+    h->omega = 2.0 * std::acos(-1.0) * c.tr_freq;
+    // control point (bubble_CUDA.cu:132-153, 214-215)
+    if (c.tr_pid && c.tr_init_control) {
+        h->ctrl_x = c.ext_focus_x_set ? h->fp_x : 0.0;
+        h->ctrl_y = c.tr_init_control;
+    } else {
+        h->ctrl_x = h->fp_x;
+        h->ctrl_y = h->fp_y;
+    }
+    h->ctrl_dist = h->ctrl_y / 0.5 / std::sqrt(3.0);
+    // init_mix, mix_set_time_increment (bubble_CUDA.cu:636-658)
+    h->T_inf = 293.15;
+    h->P_inf = 0.1e6;
+    h->rho_inf = density_water();
+    h->cs_inf = sound_speed_water();
+    h->dt = c.sim_cfl * std::min(g.dx, g.dy) / h->cs_inf;
+    h->dt0_rp = 0.1 * h->dt;
+    g.dt = h->dt;
+    g.rho_inf = h->rho_inf;
+    g.cs_inf = h->cs_inf;
+    g.P_inf = h->P_inf;
+    g.pml_x0 = c.pml_x0;
+    g.pml_x1 = c.pml_x1;
+    g.pml_y0 = c.pml_y0;
+    g.pml_y1 = c.pml_y1;
+    g.plane_v = c.tr_plane_v;
+    g.plane_p = c.tr_plane_p;
+    g.focused_v = c.tr_focused_v;
+    g.focused_p = c.tr_focused_p;
+    g.src_row = (!c.pml_y0 || c.ext_source_in_pml) ? 1 : 0;
+    g.cylindrical = c.tr_cylindrical;
+    g.amp = c.tr_amp;
+    g.freq = c.tr_freq;
+    g.omega = h->omega;
+    g.on_wave = c.tr_on_wave;
+    g.off_wave = c.tr_off_wave;
+    g.fp_x = h->ctrl_x;
+    g.fp_y = h->ctrl_y;
+    g.fdist = h->ctrl_dist;
+    g.n_elem = 0;
+    // MixturePressureKernel s3 = dt * rho_l * c * c (kernel.cuh:852)
+    g.s3 = h->dt * h->rho_inf * h->cs_inf * h->cs_inf;
+    find_aperture(h);
+    init_sigma(h);
+    // bubble parameters (init_bub_params, bubble_CUDA.cu:661-675; delta_coef 1467-1471)
+    BubbleParams &bp = h->bp;
+    bp.PL0 = c.bub_P0;
+    bp.T0 = c.bub_T0;
+    bp.gam = c.bub_gamma;
+    bp.sig = c.bub_sigma;
+    bp.mu = c.bub_mu;
+    bp.rho = c.bub_rho;
+    bp.K0 = c.bub_K0;
+    bp.dt0 = h->dt0_rp;
+    bp.mbs = -c.sim_smooth_delta_band / 2 + 1;
+    bp.mbe = c.sim_smooth_delta_band + bp.mbs - 1;
+    bp.nbs = -c.sim_smooth_delta_band / 2;
+    bp.nbe = c.sim_smooth_delta_band + bp.nbs - 1;
+    bp.dcx = 1.0 / ((double)c.sim_smooth_delta_band);
+    bp.dcy = 2.0 * std::acos(-1.0) / ((double)c.sim_smooth_delta_band) * g.rdx;
+    bp.dcz = 2.0 * std::acos(-1.0) / ((double)c.sim_smooth_delta_band) * g.rdy;
+    bp.pi = std::acos(-1.0);
+    bp.pi4r3 = bp.pi * 4.0 / 3.0;
+    bp.max_substeps = c.max_substeps > 0 ? c.max_substeps : 1000000;
+    bp.cylindrical = c.tr_cylindrical;
+
+    int rc = h->prec == BRP_PREC_FP64 ? alloc_grid<double>(h) : alloc_grid<float>(h);
+    if (rc) return fail(rc);
+    if ((rc = dalloc(h, (void **)&h->sc_d, sizeof(StepScalars)))) return fail(rc);
+    if (hipHostMalloc((void **)&h->sc_h, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess)
+        return fail(set_err(h, BRP_ERR_DEVICE, "hipHostMalloc failed"));
+    // init_bub_array (bubble_CUDA.cu:797-897): one bubble per cell centre in the box
+    if (c.bub_enable) {
+        for (int i = 1; i <= h->N; i++) {
+            const double px = ((double)i - 0.5) * g.dx;
+            for (int j = 1; j <= h->M; j++) {
+                const double py = ((double)j - 0.5) * g.dy;
+                if (c.tr_box_size ? (std::fabs(px - h->fp_x) < 0.5 * c.tr_box_size &&
+                                     std::fabs(py - h->fp_y) < 0.5 * c.tr_box_size)
+                                  : true) {
+                    h->pending_x.push_back(px);
+                    h->pending_y.push_back(py);
+                }
+            }
+        }
+        h->pending_R0.assign(h->pending_x.size(), c.bub_R0);
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(set_err(h, BRP_ERR_DEVICE, "device initialisation failed"));
+    *out = h;
+    return BRP_OK;
+}
+
+void brp_destroy(brp_handle *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &sn : h->snaps)
+        for (void *p : sn.bufs) (void)hipFree(p);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->sc_h) (void)hipHostFree(h->sc_h);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int brp_get_info(const brp_handle *h, brp_info *info) {
+    if (!h || !info) return BRP_ERR_INVALID;
+    std::memset(info, 0, sizeof *info);
+    info->nx = h->N;
+    info->ny = h->M;
+    info->n_bubbles = h->b.x ? h->nb : (int)h->pending_x.size();
+    info->precision = h->prec;
+    info->dx = h->g.dx;
+    info->dy = h->g.dy;
+    info->rdx = h->g.rdx;
+    info->rdy = h->g.rdy;
+    info->dt = h->dt;
+    info->dt0_rp = h->dt0_rp;
+    info->fp_x = h->fp_x;
+    info->fp_y = h->fp_y;
+    info->omega = h->omega;
+    info->time = h->t;
+    info->nstep = h->nstep;
+    info->device_bytes = (int64_t)h->dev_bytes;
+    return BRP_OK;
+}
+
+int brp_set_bubbles(brp_handle *h, int32_t n, const double *x, const double *y, const double *R0,
+                    const double *n_B, double fg_in) {
+    if (!h) return BRP_ERR_INVALID;
+    if (n < 0 || (n > 0 && (!x || !y || !R0))) return set_err(h, BRP_ERR_INVALID, "bad bubble arrays");
+    if (h->prepared) return set_err(h, BRP_ERR_STATE, "set_bubbles after prepare");
+    if (!h->cfg.bub_enable) return set_err(h, BRP_ERR_INVALID, "bubbles.enable is 0");
+    for (int k = 0; k < n; k++)
+        if (!(R0[k] > 0) || !std::isfinite(x[k]) || !std::isfinite(y[k]))
+            return set_err(h, BRP_ERR_INVALID, "bubble %d: R0 must be > 0, x and y finite", k);
+    h->pending_x.assign(x, x + n);
+    h->pending_y.assign(y, y + n);
+    h->pending_R0.assign(R0, R0 + n);
+    if (n_B)
+        h->pending_nB.assign(n_B, n_B + n);
+    else
+        h->pending_nB.clear();
+    h->cfg.bub_fg0 = fg_in;  // fg_in of bubble_input for these bubbles
+    return BRP_OK;
+}
+
+int brp_set_source(brp_handle *h, int32_t n_elem, const double *amp, const double *phase) {
+    if (!h) return BRP_ERR_INVALID;
+    if (n_elem < 0 || (n_elem > 0 && (!amp || !phase)))
+        return set_err(h, BRP_ERR_INVALID, "bad source arrays");
+    std::vector<double> a(amp, amp + n_elem), p(phase, phase + n_elem);
+    if (!h->el_amp_d) {
+        int rc;
+        if ((rc = dalloc(h, (void **)&h->el_amp_d, sizeof(double) * 4096)) ||
+            (rc = dalloc(h, (void **)&h->el_phase_d, sizeof(double) * 4096)))
+            return rc;
+    }
+    if (n_elem > 4096) return set_err(h, BRP_ERR_INVALID, "at most 4096 source elements");
+    if (n_elem > 0) {
+        HIPCHK(h, hipMemcpy(h->el_amp_d, a.data(), sizeof(double) * n_elem, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->el_phase_d, p.data(), sizeof(double) * n_elem, hipMemcpyHostToDevice));
+    }
+    h->n_elem = n_elem;
+    h->g.n_elem = n_elem;
+    return BRP_OK;
+}
+
+int brp_prepare(brp_handle *h) {
+    if (!h) return BRP_ERR_INVALID;
+    if (h->prepared) return BRP_OK;
+    int rc;
+    if (h->cfg.bub_enable) {
+        const std::vector<double> *nB = h->pending_nB.empty() ? nullptr : &h->pending_nB;
+        if ((rc = place_bubbles(h, h->pending_x, h->pending_y, h->pending_R0, nB, h->cfg.bub_fg0)))
+            return rc;
+        // initial window: every bubble's scatter stencil
+        for (int k = 0; k < h->nb; k++) {
+            const int ix = host_bin(h->pending_x[k], h->g.rdx, 0.5);
+            const int iy = host_bin(h->pending_y[k], h->g.rdy, 0.5);
+            window_union(h->W, std::max(ix + h->bp.mbs, 0), std::min(ix + h->bp.mbe, h->N + 1),
+                         std::max(iy + h->bp.mbs, 0), std::min(iy + h->bp.mbe, h->M + 1));
+        }
+        launch_zero_scalars(h->stream, h->sc_d, 1);
+        launch_scatter_initial(h->stream, h->g, h->bp, h->b, h->fgb[0], h->sc_d);
+        HIPCHK(h, hipMemcpyAsync(h->fgb[1], h->fgb[0], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->fgb[2], h->fgb[0], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, offsetof(StepScalars, resid), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (h->sc_h->flags) return set_err(h, BRP_ERR_NONFINITE, "initial void fraction out of range");
+    }
+    h->fcur = 0;
+    h->fn = 1;
+    h->fm = 2;
+    h->fz = 3;
+    h->prepared = true;
+    return BRP_OK;
+}
+
+int brp_step(brp_handle *h, int32_t k, brp_step_stats *stats) {
+    if (!h || k < 0) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    for (double &v : h->phase_ms) v = 0.0;
+    return h->prec == BRP_PREC_FP64 ? run_steps<double>(h, k, stats) : run_steps<float>(h, k, stats);
+}
+
+int brp_field_shape(const brp_handle *h, int32_t field, int32_t *nx, int32_t *ny, int32_t *i0,
+                    int32_t *j0) {
+    if (!h || field < 0 || field >= BRP_FIELD_COUNT) return BRP_ERR_INVALID;
+    int a, b2, c, d;
+    field_extent(h, field, &a, &b2, &c, &d);
+    if (nx) *nx = a;
+    if (ny) *ny = b2;
+    if (i0) *i0 = c;
+    if (j0) *j0 = d;
+    return BRP_OK;
+}
+
+int brp_get_field(brp_handle *h, int32_t field, double *dst, int64_t ld) {
+    if (!h || !dst || field < 0 || field >= BRP_FIELD_COUNT) return BRP_ERR_INVALID;
+    int nx, ny, i0, j0;
+    field_extent(h, field, &nx, &ny, &i0, &j0);
+    if (ld < nx) return set_err(h, BRP_ERR_INVALID, "ld %lld < %d", (long long)ld, nx);
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    std::vector<double> fr;
+    const bool dbl = h->prec == BRP_PREC_FP64;
+    auto rd = [&](const void *src) { return dbl ? read_frame<double>(h, src, fr) : read_frame<float>(h, src, fr); };
+    switch (field) {
+        case BRP_FIELD_P0: rc = rd(h->p0); break;
+        case BRP_FIELD_PX: rc = rd(h->pb[h->pc][0]); break;
+        case BRP_FIELD_PY: rc = rd(h->pb[h->pc][1]); break;
+        case BRP_FIELD_VX: rc = rd(h->vx); break;
+        case BRP_FIELD_VY: rc = rd(h->vy); break;
+        case BRP_FIELD_FG: rc = read_frame_fg(h, h->fgb[h->fcur], fr); break;
+        case BRP_FIELD_FGN: rc = read_frame_fg(h, h->fgb[h->fn], fr); break;
+        case BRP_FIELD_FGM: rc = read_frame_fg(h, h->fgb[h->fm], fr); break;
+        case BRP_FIELD_RHO_M:
+            if (h->first_step) {
+                fr.assign(h->frame, h->rho_inf);
+            } else {
+                rc = read_frame_fg(h, h->fgb[h->fcur], fr);
+                for (double &v : fr) v = (1.0 - v) * h->rho_inf;
+            }
+            break;
+        case BRP_FIELD_T: fr.assign(h->frame, 0.0); break;
+        default: return BRP_ERR_INVALID;
+    }
+    if (rc) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) dst[(size_t)j * ld + i] = fr[frame_off(i + i0, j + j0, h->pitch)];
+    return BRP_OK;
+}
+
+int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld) {
+    if (!h || !src || field < 0 || field >= BRP_FIELD_COUNT) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    int nx, ny, i0, j0;
+    field_extent(h, field, &nx, &ny, &i0, &j0);
+    if (ld < nx) return set_err(h, BRP_ERR_INVALID, "ld %lld < %d", (long long)ld, nx);
+    if (field == BRP_FIELD_FG) {
+        std::vector<int64_t> fr(h->frame, 0);
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double v = src[(size_t)j * ld + i];
+                if (!(std::fabs(v) < 1.0)) return set_err(h, BRP_ERR_INVALID, "void fraction must be in (-1, 1)");
+                const int64_t q = (int64_t)std::llrint(v * kFgScale);
+                fr[frame_off(i + i0, j + j0, h->pitch)] = q;
+                if (q != 0) window_union(h->W, i + i0, i + i0, j + j0, j + j0);
+            }
+        for (int a : {h->fcur, h->fn, h->fm})
+            HIPCHK(h, hipMemcpy(h->fgb[a], fr.data(), sizeof(int64_t) * h->frame, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemset(h->fgb[h->fz], 0, sizeof(int64_t) * h->frame));
+        return BRP_OK;
+    }
+    void *dstp = nullptr;
+    switch (field) {
+        case BRP_FIELD_P0: dstp = h->p0; break;
+        case BRP_FIELD_PX: dstp = h->pb[h->pc][0]; break;
+        case BRP_FIELD_PY: dstp = h->pb[h->pc][1]; break;
+        case BRP_FIELD_VX: dstp = h->vx; break;
+        case BRP_FIELD_VY: dstp = h->vy; break;
+        default: return set_err(h, BRP_ERR_INVALID, "field %d is not settable", field);
+    }
+    const bool dbl = h->prec == BRP_PREC_FP64;
+    std::vector<double> fr;
+    if ((rc = dbl ? read_frame<double>(h, dstp, fr) : read_frame<float>(h, dstp, fr))) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) fr[frame_off(i + i0, j + j0, h->pitch)] = src[(size_t)j * ld + i];
+    if (dbl) {
+        HIPCHK(h, hipMemcpy(dstp, fr.data(), sizeof(double) * h->frame, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(fr.begin(), fr.end());
+        HIPCHK(h, hipMemcpy(dstp, tmp.data(), sizeof(float) * h->frame, hipMemcpyHostToDevice));
+    }
+    return BRP_OK;
+}
+
+int brp_get_bubbles(brp_handle *h, int32_t which, double *dst) {
+    if (!h || !dst || which < 0 || which >= BRP_BUB_COUNT) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    const int n = h->nb;
+    if (n == 0) return BRP_OK;
+    const double *src = nullptr;
+    double *tmpd = nullptr;
+    const int c = h->rp_cur;
+    switch (which) {
+        case BRP_BUB_X: src = h->b.x; break;
+        case BRP_BUB_Y: src = h->b.y; break;
+        case BRP_BUB_R_T: src = h->b.R_t; break;
+        case BRP_BUB_R_P: src = h->b.R[c]; break;
+        case BRP_BUB_R_N: src = h->b.Rn[c]; break;
+        case BRP_BUB_D1R: src = h->b.d1R[c]; break;
+        case BRP_BUB_PG: src = h->b.PG[c]; break;
+        case BRP_BUB_PL_P: src = h->b.PL[h->pl_p]; break;
+        case BRP_BUB_PL_N: src = h->b.PL[h->pl_n]; break;
+        case BRP_BUB_PL_M: src = h->b.PL[h->pl_m]; break;
+        case BRP_BUB_Q_B: src = h->b.Q_B; break;
+        case BRP_BUB_N_B: src = h->b.n_B; break;
+        case BRP_BUB_DT: src = h->b.bdt[c]; break;
+        case BRP_BUB_RE: src = h->b.re[c]; break;
+        case BRP_BUB_VX: src = h->b.vx; break;
+        case BRP_BUB_VY: src = h->b.vy; break;
+        case BRP_BUB_R0: src = h->b.R0; break;
+        default: {
+            HIPCHK(h, hipMalloc((void **)&tmpd, sizeof(double) * n));
+            launch_bubble_index(h->stream, h->g, h->b, tmpd, which - BRP_BUB_IBM_X);
+            src = tmpd;
+        }
+    }
+    std::vector<double> v(n);
+    hipError_t e = hipMemcpyAsync(v.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (tmpd) (void)hipFree(tmpd);
+    if (e != hipSuccess) return set_err(h, BRP_ERR_DEVICE, "get_bubbles: %s", hipGetErrorString(e));
+    for (int k = 0; k < n; k++) dst[h->perm[k]] = v[k];
+    return BRP_OK;
+}
+
+int brp_save_state(brp_handle *h, int32_t slot) {
+    if (!h || slot < 0 || slot >= 16) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    Snapshot &sn = h->snaps[slot];
+    if (!sn.valid) {
+        for (size_t k = 0; k < h->allocs.size(); k++) {
+            void *p = nullptr;
+            if (hipMalloc(&p, h->alloc_sizes[k]) != hipSuccess) {
+                for (void *q : sn.bufs) (void)hipFree(q);
+                sn.bufs.clear();
+                sn.sizes.clear();
+                return set_err(h, BRP_ERR_DEVICE, "snapshot allocation failed");
+            }
+            sn.bufs.push_back(p);
+            sn.sizes.push_back(h->alloc_sizes[k]);
+        }
+    }
+    if (sn.bufs.size() != h->allocs.size()) return set_err(h, BRP_ERR_STATE, "snapshot layout changed");
+    for (size_t k = 0; k < h->allocs.size(); k++)
+        HIPCHK(h, hipMemcpyAsync(sn.bufs[k], h->allocs[k], sn.sizes[k], hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    sn.pc = h->pc;
+    sn.fcur = h->fcur;
+    sn.fn = h->fn;
+    sn.fm = h->fm;
+    sn.fz = h->fz;
+    sn.rp_cur = h->rp_cur;
+    sn.pl_m = h->pl_m;
+    sn.pl_n = h->pl_n;
+    sn.pl_p = h->pl_p;
+    sn.first_step = h->first_step;
+    sn.W = h->W;
+    sn.nstep = h->nstep;
+    sn.t = h->t;
+    sn.tx = h->tx;
+    sn.valid = true;
+    return BRP_OK;
+}
+
+int brp_restore_state(brp_handle *h, int32_t slot) {
+    if (!h || slot < 0 || slot >= 16) return BRP_ERR_INVALID;
+    Snapshot &sn = h->snaps[slot];
+    if (!sn.valid) return set_err(h, BRP_ERR_STATE, "slot %d holds no state", slot);
+    if (sn.bufs.size() != h->allocs.size()) return set_err(h, BRP_ERR_STATE, "snapshot layout changed");
+    for (size_t k = 0; k < h->allocs.size(); k++)
+        HIPCHK(h, hipMemcpyAsync(h->allocs[k], sn.bufs[k], sn.sizes[k], hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->pc = sn.pc;
+    h->fcur = sn.fcur;
+    h->fn = sn.fn;
+    h->fm = sn.fm;
+    h->fz = sn.fz;
+    h->rp_cur = sn.rp_cur;
+    h->pl_m = sn.pl_m;
+    h->pl_n = sn.pl_n;
+    h->pl_p = sn.pl_p;
+    h->first_step = sn.first_step;
+    h->W = sn.W;
+    h->nstep = sn.nstep;
+    h->t = sn.t;
+    h->tx = sn.tx;
+    return BRP_OK;
+}
+
+int brp_get_phase_times(const brp_handle *h, double ms[5]) {
+    if (!h || !ms) return BRP_ERR_INVALID;
+    for (int k = 0; k < 5; k++) ms[k] = h->phase_ms[k];
+    return BRP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// brp_internal.h -- device data layout and kernel entry points of
+// libbubbles_rp (the MI355X coupled FDTD + Rayleigh-Plesset step).
+//
+// HBM layout
+//   Every grid array lives in one common frame: logical columns
+//   i in [-1, N+1] and rows j in [-1, M+1] (the union of the reference's
+//   1m, 2m, 2n extents for simulation.order = 2), row-major, element
+//   (i, j) at  (j + 1) * pitch + (i + kCol0).  kCol0 = 64 puts column i = 0
+//   on a 256-byte boundary for both float and double, so a wavefront's 64
+//   consecutive cells are one aligned 256/512-byte segment.  pitch is a
+//   multiple of 64 elements.
+//   Fields: vx, vy, p.x, p.y (two buffers, rotated as p / pn), p0 in the
+//   precision's field type F (double or float); void fraction in four int64
+//   fixed-point buffers (value * 2^60) rotated as f_g / f_gn / f_gm / spare.
+//   Bubbles: structure of arrays in FP64, sorted by cell at set-up.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bubbles_rp.h"
+
+namespace brp {
+
+constexpr int kCol0 = 64;
+constexpr double kFgScale = 1152921504606846976.0;       // 2^60
+constexpr double kFgInvScale = 8.673617379884035e-19;    // 2^-60
+constexpr int kMaxEval = 1024;                           // residual slots per step
+
+// Constant parameters of the grid kernels (passed by value).
+struct GridParams {
+    int N, M;            // interior cells in x and y
+    int pitch;           // elements per frame row
+    double dx, dy, rdx, rdy;
+    double dt;
+    double rho_inf, cs_inf;
+    double P_inf;
+    int pml_x0, pml_x1, pml_y0, pml_y1;
+    int plane_v, plane_p, focused_v, focused_p;
+    int src_row;         // source row written (pml.y0 == 0 or extension E1)
+    int cylindrical;
+    double amp, freq, omega;
+    int on_wave, off_wave;
+    double fp_x, fp_y, fdist;  // control point and focal distance
+    int ap_lo, ap_hi, n_elem;  // aperture cells and element count (E4)
+    double s3;           // dt * rho_l * c * c   (MixturePressureKernel)
+};
+
+__host__ __device__ inline int64_t frame_off(int i, int j, int pitch) {
+    return (int64_t)(j + 1) * pitch + (i + kCol0);
+}
+
+// Window of cells where any void-fraction buffer may be nonzero.
+struct Window {
+    int i0, i1, j0, j1;  // inclusive, within [0, N+1]
+};
+
+// Device-side per-step scalars written by the kernels and read by the host.
+struct StepScalars {
+    int bbox[4];                         // min i, max i, min j, max j of bubble stencils
+    int flags;                           // bit 0 nonfinite p / f_g >= 1, bit 1 scatter
+                                         // range / nonfinite R, bit 2 sub-step cap
+    int sub_max;                         // max RP sub-steps this step
+    unsigned long long sub_sum;          // sum of RP sub-steps this step
+    unsigned long long resid[kMaxEval];  // max |dp0| bit patterns (>= 0 doubles)
+};
+
+// Bubble arrays (device pointers; n padded to a multiple of 256).
+struct BubbleArrays {
+    int n;
+    double *x, *y;
+    double *R[2], *Rn[2], *d1R[2], *PG[2], *bdt[2], *re[2];  // RP state, double-buffered
+    double *R_t, *Q_B;
+    double *PL[3];        // rotating liquid-pressure history
+    double *n_B, *R0, *PG0, *calpha;
+    double *vx, *vy;      // v_B (= v_L)
+    int *sub;             // sub-steps of the last RP call, per bubble
+};
+
+// Bubble physics constants shared by every bubble.
+struct BubbleParams {
+    double PL0, T0, gam, sig, mu, rho, K0, dt0;
+    int mbs, mbe, nbs, nbe;
+    double dcx, dcy, dcz;   // delta_coef
+    double pi4r3, pi;
+    int max_substeps;
+    int cylindrical;
+};
+
+}  // namespace brp
+
+// Kernel launchers (kernels_*.hip).  All run on stream s.
+namespace brp {
+template <typename F>
+void launch_velocity(hipStream_t s, const GridParams &g, F *vx, F *vy, const F *p0,
+                     const int64_t *fgn, Window w, int first_step, const F *cvx_a,
+                     const F *cvx_b, const F *cvy_a, const F *cvy_b);
+template <typename F>
+void launch_velocity_ghosts(hipStream_t s, const GridParams &g, F *vx, F *vy, double t);
+template <typename F>
+void launch_pressure_first(hipStream_t s, const GridParams &g, const F *vx, const F *vy,
+                           const F *pnx, const F *pny, F *px, F *py, F *p0,
+                           const int64_t *fg_cur, const int64_t *fg_n, int64_t *fg_mp,
+                           Window w, const F *ax, const F *bx, const F *ay, const F *by,
+                           const double *xu, const double *rxp, StepScalars *sc);
+template <typename F>
+void launch_pressure_picard(hipStream_t s, const GridParams &g, const F *vx, const F *vy,
+                            const F *pnx, const F *pny, F *px, F *py, F *p0,
+                            const int64_t *fg_k, const int64_t *fg_n, int64_t *fg_prev,
+                            Window w, const F *ax, const F *bx, const F *ay, const F *by,
+                            const double *xu, const double *rxp, StepScalars *sc, int slot);
+template <typename F>
+void launch_pressure_ghosts(hipStream_t s, const GridParams &g, F *p0, const double *el_amp,
+                            const double *el_phase, double t);
+template <typename F>
+void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                   const BubbleArrays &b, const F *vx, const F *vy, StepScalars *sc);
+template <typename F>
+void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                        const BubbleArrays &b, int cur, int plm, int pln, int plp,
+                        const F *p0, int64_t *fg_target, StepScalars *sc, int do_rp);
+void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                            const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
+void launch_copy_d(hipStream_t s, double *dst, const double *src, int n);
+void launch_bubble_index(hipStream_t s, const GridParams &g, const BubbleArrays &b,
+                         double *out, int which);
+void launch_zero_scalars(hipStream_t s, StepScalars *sc, int nslots);
+}  // namespace brp

@@ -1,0 +1,333 @@
+// kernels_bubble.hip -- Lagrangian bubble kernels for gfx950.
+//
+// One bubble per lane.  Bubbles are stored sorted by cell, so the B x B
+// scatter stencils of neighbouring lanes hit the same few cells; the
+// void-fraction accumulation is int64 fixed point (value * 2^60): integer
+// atomics are associative, so the scatter is bit-reproducible in any order
+// (the FP64 CAS loop of the reference, bubble_CUDA_kernel.cuh:95-114, is not).
+#include "brp_internal.h"
+#include "rp_device.h"
+
+#include <math.h>
+
+namespace brp {
+namespace {
+
+constexpr int kMaxBand = 8;
+
+// Bit-exact binning: floor(x * rdx + c) with no FMA contraction
+// (BubbleUpdateIndexKernel, bubble_CUDA_kernel.cuh:355-356).
+__device__ inline int bin_index(double x, double rdx, double c) {
+#pragma clang fp contract(off)
+    double v = floor(x * rdx + c);
+    return (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
+}
+
+// smooth_delta_x / smooth_delta_y (bubble_CUDA_kernel.cuh:214-222)
+__device__ inline double delta_w(double coef_x, double coef_k, int pos, double h, double b) {
+#pragma clang fp contract(off)
+    return coef_x * (1.0 + cos(coef_k * (((double)pos - 0.5) * h - b)));
+}
+
+__device__ inline int64_t to_fixed(double v, bool *bad) {
+    double s = v * kFgScale;
+    if (!(fabs(s) < 4.611686018427388e18)) {  // |v| < 4, NaN rejected
+        *bad = true;
+        return 0;
+    }
+    return (int64_t)__double2ll_rn(s);
+}
+
+// Block-wide reduction of (min, max) pairs into sc->bbox with four atomics.
+__device__ inline void block_bbox(int imin, int imax, int jmin, int jmax, StepScalars *sc) {
+    for (int off = 32; off > 0; off >>= 1) {
+        imin = min(imin, __shfl_xor(imin, off, 64));
+        imax = max(imax, __shfl_xor(imax, off, 64));
+        jmin = min(jmin, __shfl_xor(jmin, off, 64));
+        jmax = max(jmax, __shfl_xor(jmax, off, 64));
+    }
+    __shared__ int red[4][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = imin;
+        red[1][w] = imax;
+        red[2][w] = jmin;
+        red[3][w] = jmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int k = 1; k < nw; k++) {
+            imin = min(imin, red[0][k]);
+            imax = max(imax, red[1][k]);
+            jmin = min(jmin, red[2][k]);
+            jmax = max(jmax, red[3][k]);
+        }
+        if (imin <= imax) {
+            atomicMin(&sc->bbox[0], imin);
+            atomicMax(&sc->bbox[1], imax);
+            atomicMin(&sc->bbox[2], jmin);
+            atomicMax(&sc->bbox[3], jmax);
+        }
+    }
+}
+
+// BubbleInterpolationVelocityKernel + v_B <- v_L + BubbleMotionKernel
+// (bubble_CUDA_kernel.cuh:398-468, 1282-1297), fused per bubble; also the
+// bounding box of the new scatter stencils for the Picard window.
+template <typename F>
+__global__ __launch_bounds__(256) void k_motion(GridParams g, BubbleParams bp, BubbleArrays b,
+                                                 const F *__restrict__ vx,
+                                                 const F *__restrict__ vy, StepScalars *sc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int imin = 0x7fffffff, imax = -0x7fffffff, jmin = 0x7fffffff, jmax = -0x7fffffff;
+    if (idx < b.n) {
+        const double x = b.x[idx], y = b.y[idx];
+        const int ibmx = bin_index(x, g.rdx, 1.0), ibmy = bin_index(y, g.rdy, 1.0);
+        const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+        double vLx = 0.0, vLy = 0.0;
+        for (int i = ibmx + bp.nbs; i <= ibmx + bp.nbe; i++) {
+            const double Dx = delta_w(bp.dcx, bp.dcy, i, g.dx, x);
+            for (int j = ibny + bp.mbs; j <= ibny + bp.mbe; j++) {
+                const double Dy = delta_w(bp.dcx, bp.dcz, j, g.dy, y);
+                if (i >= -1 && i <= g.N + 1 && j >= 0 && j <= g.M + 1)
+                    vLx += (double)vx[frame_off(i, j, g.pitch)] * Dx * Dy;
+            }
+        }
+        for (int i = ibnx + bp.mbs; i <= ibnx + bp.mbe; i++) {
+            const double Dx = delta_w(bp.dcx, bp.dcy, i, g.dx, x);
+            for (int j = ibmy + bp.nbs; j <= ibmy + bp.nbe; j++) {
+                const double Dy = delta_w(bp.dcx, bp.dcz, j, g.dy, y);
+                if (i >= 0 && i <= g.N + 1 && j >= -1 && j <= g.M + 1)
+                    vLy += (double)vy[frame_off(i, j, g.pitch)] * Dx * Dy;
+            }
+        }
+        b.vx[idx] = vLx;
+        b.vy[idx] = vLy;
+        double nx, ny;
+        {
+#pragma clang fp contract(off)
+            nx = x + vLx * g.dt;
+            ny = y + vLy * g.dt;
+        }
+        b.x[idx] = nx;
+        b.y[idx] = ny;
+        const int kx = bin_index(nx, g.rdx, 0.5), ky = bin_index(ny, g.rdy, 0.5);
+        imin = max(kx + bp.mbs, 0);
+        imax = min(kx + bp.mbe, g.N + 1);
+        jmin = max(ky + bp.mbs, 0);
+        jmax = min(ky + bp.mbe, g.M + 1);
+        if (imin > imax || jmin > jmax) {
+            imin = 0x7fffffff;
+            imax = -0x7fffffff;
+            jmin = 0x7fffffff;
+            jmax = -0x7fffffff;
+        }
+    }
+    block_bbox(imin, imax, jmin, jmax, sc);
+}
+
+// The scatter of one bubble's volume (VoidFractionReverseLookupKernel,
+// bubble_CUDA_kernel.cuh:576-619) given its stencil weights.
+__device__ inline void scatter_one(const GridParams &g, const BubbleParams &bp, int ibnx,
+                                   int ibny, const double *wx, const double *wy, double fg_temp,
+                                   int64_t *fg, bool *bad) {
+    const int nb = bp.mbe - bp.mbs + 1;
+    for (int a = 0; a < nb; a++) {
+        const int i = ibnx + bp.mbs + a;
+        for (int c = 0; c < nb; c++) {
+            const int j = ibny + bp.mbs + c;
+            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+                double v;
+                {
+#pragma clang fp contract(off)
+                    v = fg_temp * wx[a] * wy[c];
+                }
+                const int64_t q = to_fixed(v, bad);
+                if (q != 0)
+                    atomicAdd((unsigned long long *)&fg[frame_off(i, j, g.pitch)],
+                              (unsigned long long)q);
+            }
+        }
+    }
+}
+
+__device__ inline double fg_temp_of(const GridParams &g, const BubbleParams &bp, double n_B,
+                                    double R, double x) {
+#pragma clang fp contract(off)
+    double v = bp.pi4r3 * n_B * (R * R * R) * g.rdx * g.rdy;
+    if (bp.cylindrical) v = bp.pi4r3 * n_B * (R * R * R) * g.rdx * g.rdy / x;
+    return v;
+}
+
+// One Picard iteration for every bubble, fused:
+//   BubbleInterpolationScalarKernel (gather PL_p, kernel.cuh:361-395)
+//   BubbleRadiusKernel              (RP from the step's start state, 471-554)
+//   VoidFractionReverseLookupKernel (scatter R_t into fg_target, 576-619)
+template <typename F>
+__global__ __launch_bounds__(256) void k_bubble_step(GridParams g, BubbleParams bp,
+                                                      BubbleArrays b, int cur, int plm, int pln,
+                                                      int plp, const F *__restrict__ p0,
+                                                      int64_t *__restrict__ fg_target,
+                                                      StepScalars *sc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int sub = 0;
+    bool bad = false, capped = false;
+    if (idx < b.n) {
+        const double x = b.x[idx], y = b.y[idx];
+        const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+        const int nb = bp.mbe - bp.mbs + 1;
+        double wx[kMaxBand], wy[kMaxBand];
+        for (int a = 0; a < nb; a++) {
+            wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
+            wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
+        }
+        double PL = 0.0;
+        for (int a = 0; a < nb; a++) {
+            const int i = ibnx + bp.mbs + a;
+            for (int c = 0; c < nb; c++) {
+                const int j = ibny + bp.mbs + c;
+                if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+#pragma clang fp contract(off)
+                    PL += (double)p0[frame_off(i, j, g.pitch)] * wx[a] * wy[c];
+                }
+            }
+        }
+        b.PL[plp][idx] = PL;
+        const int nxt = cur ^ 1;
+        RPState in{b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx],
+                   b.PG[cur][idx], b.bdt[cur][idx], b.re[cur][idx]};
+        RPConst k{b.R0[idx], b.R0[idx] * b.R0[idx] * b.R0[idx], b.PG0[idx], b.calpha[idx]};
+        RPShared s{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
+        RPState out;
+        double R_t, Q_B;
+        sub = rp_step(in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, k, s, bp.max_substeps,
+                      &out, &R_t, &Q_B, &capped);
+        b.R[nxt][idx] = out.Rp;
+        b.Rn[nxt][idx] = out.Rn;
+        b.d1R[nxt][idx] = out.d1R;
+        b.PG[nxt][idx] = out.PG;
+        b.bdt[nxt][idx] = out.dt;
+        b.re[nxt][idx] = out.re;
+        b.R_t[idx] = R_t;
+        b.Q_B[idx] = Q_B;
+        b.sub[idx] = sub;
+        bad |= !isfinite(R_t);
+        scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp_of(g, bp, b.n_B[idx], R_t, x), fg_target,
+                    &bad);
+    }
+    // per-step statistics: max and sum of sub-steps, error flags
+    int smax = sub;
+    unsigned long long ssum = (unsigned long long)sub;
+    for (int off = 32; off > 0; off >>= 1) {
+        smax = max(smax, __shfl_xor(smax, off, 64));
+        ssum += __shfl_xor(ssum, off, 64);
+    }
+    __shared__ int rmax[4];
+    __shared__ unsigned long long rsum[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        rmax[w] = smax;
+        rsum[w] = ssum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k2 = 1; k2 < (int)(blockDim.x >> 6); k2++) {
+            smax = max(smax, rmax[k2]);
+            ssum += rsum[k2];
+        }
+        if (ssum) atomicAdd(&sc->sub_sum, ssum);
+        if (smax > __hip_atomic_load(&sc->sub_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&sc->sub_max, smax);
+    }
+    if (bad) atomicOr(&sc->flags, 2);
+    if (capped) atomicOr(&sc->flags, 4);
+}
+
+// The initial void fraction (calculate_void_fraction at set-up,
+// bubble_CUDA.cu:180-185): scatter of R_t = R0.
+__global__ __launch_bounds__(256) void k_scatter_initial(GridParams g, BubbleParams bp,
+                                                          BubbleArrays b, int64_t *fg,
+                                                          StepScalars *sc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (idx < b.n) {
+        const double x = b.x[idx], y = b.y[idx];
+        const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+        const int nb = bp.mbe - bp.mbs + 1;
+        double wx[kMaxBand], wy[kMaxBand];
+        for (int a = 0; a < nb; a++) {
+            wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
+            wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
+        }
+        scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp_of(g, bp, b.n_B[idx], b.R_t[idx], x), fg,
+                    &bad);
+    }
+    if (bad) atomicOr(&sc->flags, 2);
+}
+
+__global__ void k_copy_d(double *dst, const double *src, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+// Exports ibm / ibn (which = 0 ibm.x, 1 ibm.y, 2 ibn.x, 3 ibn.y) as doubles.
+__global__ void k_bubble_index(GridParams g, BubbleArrays b, double *out, int which) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= b.n) return;
+    const double v = (which & 1) ? b.y[idx] : b.x[idx];
+    const double r = (which & 1) ? g.rdy : g.rdx;
+    out[idx] = (double)bin_index(v, r, which < 2 ? 1.0 : 0.5);
+}
+
+}  // namespace
+
+template <typename F>
+void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                   const BubbleArrays &b, const F *vx, const F *vy, StepScalars *sc) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_motion<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, vx, vy,
+                       sc);
+}
+
+template <typename F>
+void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                        const BubbleArrays &b, int cur, int plm, int pln, int plp, const F *p0,
+                        int64_t *fg_target, StepScalars *sc, int) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_bubble_step<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, cur,
+                       plm, pln, plp, p0, fg_target, sc);
+}
+
+void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                            const BubbleArrays &b, int64_t *fg_target, StepScalars *sc) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_initial, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b,
+                       fg_target, sc);
+}
+
+void launch_copy_d(hipStream_t s, double *dst, const double *src, int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_d, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+}
+
+void launch_bubble_index(hipStream_t s, const GridParams &g, const BubbleArrays &b, double *out,
+                         int which) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_bubble_index, dim3((b.n + 255) / 256), dim3(256), 0, s, g, b, out, which);
+}
+
+template void launch_motion<double>(hipStream_t, const GridParams &, const BubbleParams &,
+                                    const BubbleArrays &, const double *, const double *,
+                                    StepScalars *);
+template void launch_motion<float>(hipStream_t, const GridParams &, const BubbleParams &,
+                                   const BubbleArrays &, const float *, const float *,
+                                   StepScalars *);
+template void launch_bubble_step<double>(hipStream_t, const GridParams &, const BubbleParams &,
+                                         const BubbleArrays &, int, int, int, int,
+                                         const double *, int64_t *, StepScalars *, int);
+template void launch_bubble_step<float>(hipStream_t, const GridParams &, const BubbleParams &,
+                                        const BubbleArrays &, int, int, int, int, const float *,
+                                        int64_t *, StepScalars *, int);
+
+}  // namespace brp
