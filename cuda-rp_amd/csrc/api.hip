@@ -422,7 +422,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
         std::memcpy(&r0, &h->sc_h->resid[0], 8);
         double resimax = r0 / h->P_inf;
         st.resimax0 = resimax;
-        if (bub) window_union(h->W, h->sc_h->bbox[0], h->sc_h->bbox[1], h->sc_h->bbox[2], h->sc_h->bbox[3]);
+        if (bub) {
+            window_union(h->W, h->sc_h->bbox[0], h->sc_h->bbox[1], h->sc_h->bbox[2], h->sc_h->bbox[3]);
+            if (h->cfg.tr_cylindrical && h->W.i0 <= 1 && h->W.i0 <= h->W.i1) h->W.i0 = 0;  // axis fold target
+        }
         int status = BRP_OK;
         int loop = 0;
         int target = h->fz, prev = old_m;
@@ -437,6 +440,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
                 launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
                                       p0, h->fgb[target], h->sc_d, 1);
+                if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
                 HIPCHK(h, hipEventRecord(h->ev[5], s));
                 launch_pressure_picard<F>(s, h->g, vx, vy, (F *)h->pb[pn_i][0], (F *)h->pb[pn_i][1],
                                           (F *)h->pb[p_i][0], (F *)h->pb[p_i][1], p0,
@@ -784,7 +788,9 @@ int brp_prepare(brp_handle *h) {
                          std::max(iy + h->bp.mbs, 0), std::min(iy + h->bp.mbe, h->M + 1));
         }
         launch_zero_scalars(h->stream, h->sc_d, 1);
+        if (h->cfg.tr_cylindrical && h->W.i0 <= 1 && h->W.i0 <= h->W.i1) h->W.i0 = 0;
         launch_scatter_initial(h->stream, h->g, h->bp, h->b, h->fgb[0], h->sc_d);
+        if (h->cfg.tr_cylindrical) launch_cyl_fold(h->stream, h->g, h->fgb[0]);
         HIPCHK(h, hipMemcpyAsync(h->fgb[1], h->fgb[0], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(h, hipMemcpyAsync(h->fgb[2], h->fgb[0], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, offsetof(StepScalars, resid), hipMemcpyDeviceToHost, h->stream));
@@ -862,7 +868,7 @@ int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld) {
     int nx, ny, i0, j0;
     field_extent(h, field, &nx, &ny, &i0, &j0);
     if (ld < nx) return set_err(h, BRP_ERR_INVALID, "ld %lld < %d", (long long)ld, nx);
-    if (field == BRP_FIELD_FG) {
+    if (field == BRP_FIELD_FG || field == BRP_FIELD_FGN || field == BRP_FIELD_FGM) {
         std::vector<int64_t> fr(h->frame, 0);
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
@@ -872,8 +878,13 @@ int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld) {
                 fr[frame_off(i + i0, j + j0, h->pitch)] = q;
                 if (q != 0) window_union(h->W, i + i0, i + i0, j + j0, j + j0);
             }
-        for (int a : {h->fcur, h->fn, h->fm})
+        if (field == BRP_FIELD_FG) {
+            for (int a : {h->fcur, h->fn, h->fm})
+                HIPCHK(h, hipMemcpy(h->fgb[a], fr.data(), sizeof(int64_t) * h->frame, hipMemcpyHostToDevice));
+        } else {
+            const int a = field == BRP_FIELD_FGN ? h->fn : h->fm;
             HIPCHK(h, hipMemcpy(h->fgb[a], fr.data(), sizeof(int64_t) * h->frame, hipMemcpyHostToDevice));
+        }
         HIPCHK(h, hipMemset(h->fgb[h->fz], 0, sizeof(int64_t) * h->frame));
         return BRP_OK;
     }

@@ -121,6 +121,7 @@ void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &
                         const F *p0, int64_t *fg_target, StepScalars *sc, int do_rp);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
+void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);
 void launch_copy_d(hipStream_t s, double *dst, const double *src, int n);
 void launch_bubble_index(hipStream_t s, const GridParams &g, const BubbleArrays &b,
                          double *out, int which);

@@ -266,6 +266,17 @@ __global__ __launch_bounds__(256) void k_scatter_initial(GridParams g, BubblePar
     if (bad) atomicOr(&sc->flags, 2);
 }
 
+// VoidFractionCylinderKernel (bubble_CUDA_kernel.cuh:559-573) for order 2:
+// the axis fold f_g(0, j) += f_g(1, j); f_g(1, j) = f_g(0, j), exact in int64.
+__global__ void k_cyl_fold(GridParams g, int64_t *fg) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > g.M + 1) return;
+    const int64_t o0 = frame_off(0, j, g.pitch), o1 = frame_off(1, j, g.pitch);
+    const int64_t v = fg[o0] + fg[o1];
+    fg[o0] = v;
+    fg[o1] = v;
+}
+
 __global__ void k_copy_d(double *dst, const double *src, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) dst[k] = src[k];
@@ -304,6 +315,10 @@ void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubblePara
     if (b.n <= 0) return;
     hipLaunchKernelGGL(k_scatter_initial, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b,
                        fg_target, sc);
+}
+
+void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg) {
+    hipLaunchKernelGGL(k_cyl_fold, dim3((g.M + 2 + 255) / 256), dim3(256), 0, s, g, fg);
 }
 
 void launch_copy_d(hipStream_t s, double *dst, const double *src, int n) {

@@ -226,8 +226,8 @@ int brp_field_shape(const brp_handle *h, int32_t field, int32_t *nx,
 int brp_get_field(brp_handle *h, int32_t field, double *dst, int64_t ld);
 
 /* Overwrites a field from a caller buffer.  BRP_FIELD_FG sets f_g, f_gn and
- * f_gm together (the initial void-fraction map); P0/PX/PY/VX/VY set that
- * array.  Must follow brp_prepare. */
+ * f_gm together (the initial void-fraction map); FGN, FGM, P0, PX, PY, VX
+ * and VY set that array alone.  Runs brp_prepare first if needed. */
 int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld);
 
 /* Copies one bubble array (n_bubbles values, in the order given to

@@ -1236,6 +1236,47 @@ int oracle_set_field(oracle *o, int32_t field, const double *src, int64_t ld) {
     return BRP_OK;
 }
 
+// Gather of the current p0 at every bubble (BubbleInterpolationScalarKernel)
+// without advancing anything; PL_out receives n_bubbles values.
+int oracle_gather_p0(oracle *o, double *PL_out) {
+    if (!o || !PL_out) return BRP_ERR_INVALID;
+    if (!o->prepared) oracle_prepare(o);
+    std::vector<double> keep = o->PL_p;
+    interpolate_bubble_pressure(o);
+    std::memcpy(PL_out, o->PL_p.data(), sizeof(double) * o->nb);
+    o->PL_p = keep;
+    return BRP_OK;
+}
+
+// Scatter of per-bubble volumes q (as fg_temp) with the stencil of
+// VoidFractionReverseLookupKernel into fg_out (2m layout), state untouched.
+int oracle_scatter_values(oracle *o, const double *q, double *fg_out) {
+    if (!o || !q || !fg_out) return BRP_ERR_INVALID;
+    if (!o->prepared) oracle_prepare(o);
+    const ArrayIndex &a = o->a;
+    Grid g;
+    g.init(a.ista2m, a.iend2m, a.jsta2m, a.jend2m, 0.0);
+    for (int b = 0; b < o->nb; b++) {
+        for (int i = o->ibnx[b] + o->mbs; i <= o->ibnx[b] + o->mbe; i++) {
+            double Dx = delta_x(o, i, o->bx[b]);
+            for (int j = o->ibny[b] + o->mbs; j <= o->ibny[b] + o->mbe; j++) {
+                double Dy = delta_y(o, j, o->by[b]);
+                if (i >= a.ista2m && i <= a.iend2m && j >= a.jsta2m && j <= a.jend2m)
+                    g(i, j) += q[b] * Dx * Dy;
+            }
+        }
+    }
+    std::memcpy(fg_out, g.v.data(), sizeof(double) * g.v.size());
+    return BRP_OK;
+}
+
+// One solveRayleighPlesset call.  io = {Rt, Rp, Rn, d1R, dt, remain} in/out.
+void oracle_rp_radius(const oracle_bubble_params *p, double PG, double PL, double io[6]) {
+    BubParams b{p->R0, p->R03, p->PL0, p->PG0, p->T0, p->gam, p->sig,
+                p->mu, p->rho, p->K0,  p->coeff_alpha, p->dt0};
+    solveRayleighPlesset(&io[0], &io[1], &io[2], &io[3], PG, PL, &io[4], &io[5], b);
+}
+
 int oracle_get_bubbles(const oracle *o, int32_t which, double *dst) {
     if (!o || !dst) return BRP_ERR_INVALID;
     const std::vector<double> *v = nullptr;

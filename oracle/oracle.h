@@ -36,6 +36,10 @@ int oracle_field_shape(const oracle *o, int32_t field, int32_t *nx,
 int oracle_get_field(const oracle *o, int32_t field, double *dst, int64_t ld);
 int oracle_set_field(oracle *o, int32_t field, const double *src, int64_t ld);
 int oracle_get_bubbles(const oracle *o, int32_t which, double *dst);
+/* Probes for the pinning tests (no state change): the p0 gather at every
+ * bubble, the scatter of given per-bubble volumes, one radius update. */
+int oracle_gather_p0(oracle *o, double *PL_out);
+int oracle_scatter_values(oracle *o, const double *q, double *fg_out);
 const char *oracle_last_error(const oracle *o);
 /* OpenMP threads used by oracle_step (1 = serial; result is identical). */
 void oracle_set_threads(oracle *o, int32_t n);
@@ -73,6 +77,8 @@ int oracle_rp_integrate(const oracle_bubble_params *bp, double dt_mix,
 /* Total FP64 operation count of the last oracle_rp_integrate call, by the
  * counting rule of oracle.cpp (add/sub/mul/div/sqrt = 1, fma n/a). */
 int64_t oracle_rp_flops_last(void);
+/* One solveRayleighPlesset call; io = {Rt, Rp, Rn, d1R, dt, remain}. */
+void oracle_rp_radius(const oracle_bubble_params *bp, double PG, double PL, double io[6]);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,9 @@ def load():
         "oracle_bubble_params_init": (None, [vp] + [C.c_double] * 9),
         "oracle_rp_integrate": (C.c_int, [vp, C.c_double, dp, dp, i32]),
         "oracle_rp_flops_last": (C.c_int64, []),
+        "oracle_gather_p0": (C.c_int, [vp, dp]),
+        "oracle_scatter_values": (C.c_int, [vp, dp, dp]),
+        "oracle_rp_radius": (None, [vp, C.c_double, C.c_double, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -121,6 +124,18 @@ class OracleSim(brp._Backend):
     def smooth_delta(self, axis, pos, b):
         return self.lib.oracle_smooth_delta(self.h, axis, pos, b)
 
+    def gather_p0(self):
+        out = np.zeros(self.info()["n_bubbles"])
+        self._check(self.lib.oracle_gather_p0(self.h, _d(out)))
+        return out
+
+    def scatter_values(self, q):
+        nx, ny, _, _ = self.shape("fg")
+        out = np.zeros((ny, nx))
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._check(self.lib.oracle_scatter_values(self.h, _d(q), _d(out)))
+        return out
+
     def close(self):
         if self.h:
             self.lib.oracle_destroy(self.h)
@@ -168,3 +183,10 @@ def rp_integrate(bp, dt_mix, state, max_substeps=0):
 
 def rp_flops_last():
     return load().oracle_rp_flops_last()
+
+
+def rp_radius(bp, PG, PL, io):
+    """One solveRayleighPlesset call; io = [Rt, Rp, Rn, d1R, dt, remain]."""
+    a = np.ascontiguousarray(io, dtype=np.float64)
+    load().oracle_rp_radius(C.byref(bp), PG, PL, _d(a))
+    return a

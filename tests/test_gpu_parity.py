@@ -1,0 +1,219 @@
+"""Parity of libbubbles_rp (HIP, MI355X) with the FP64 oracle through the C ABI.
+
+Tolerances (relative L2 unless stated):
+  FP64 mode, bubbles off        1e-12   (only sin/cos last-ulp differences)
+  FP64 mode, cavitating bubbles 1e-9    (+ int64 fixed-point void fraction)
+  mixed mode (FP32 fields)      1e-4    pressure; 1e-4 bubble radius
+  binning / bubble indices      bit-exact on identical FP64 positions
+  Picard iteration counts       identical per step
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as ol
+from cases import rel_l2, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("p0", "px", "py", "vx", "vy")
+
+
+def run_pair(brp, cfg, bubbles=None, steps=100, source=None):
+    g = cases.make(brp.Simulation, cfg, bubbles, source)
+    o = cases.make(ol.OracleSim, cfg, bubbles, source)
+    gs = g.step(steps)
+    os_ = o.step(steps)
+    return g, o, gs, os_
+
+
+def report(tag, g, o, names):
+    out = {}
+    for n in names:
+        a, b = g.field(n), o.field(n)
+        out[n] = rel_l2(a, b)
+    print(tag, {k: f"{v:.2e}" for k, v in out.items()}, flush=True)
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-4)])
+def test_acoustic_focused_source(brp, prec, tol):
+    cfg = cases.acoustic_case(64, precision=prec)
+    g, o, gs, os_ = run_pair(brp, cfg, steps=150)
+    err = report(f"acoustic prec={prec}", g, o, FIELDS)
+    assert np.abs(o.field("p0")).max() > 1e5  # the wave is there
+    for n in ("p0", "vx", "vy"):
+        assert err[n] < tol, n
+    for n in ("px", "py"):
+        assert err[n] < 10 * tol, n
+    assert [s["resimax0"] > 0 for s in gs] == [s["resimax0"] > 0 for s in os_]
+
+
+@pytest.mark.parametrize("cyl", [0, 1])
+def test_sample_boundaries_fp64(brp, cyl):
+    """confs/Sample.txt's boundary set: mirrors on x0/y0, PML on x1/y1, the
+    source row written because pml.y0 = 0; optionally cylindrical."""
+    cfg = cases.sample_like_case(48, cylindrical=cyl)
+    g, o, gs, os_ = run_pair(brp, cfg, steps=150)
+    err = report(f"sample cyl={cyl}", g, o, FIELDS + ("fg",))
+    for n in FIELDS:
+        assert err[n] < 1e-12, n
+    assert err["fg"] < 1e-10
+    assert [s["n_pic"] for s in gs] == [s["n_pic"] for s in os_]
+    for n in ("R_t", "x", "y"):
+        assert rel_l2(g.bubbles(n), o.bubbles(n)) < 1e-12
+
+
+def test_all_mirror_walls_and_plane_sources(brp):
+    """No PML at all: every mirror branch of VelocityBoundaryKernel and
+    MixtureBoundaryPressureKernel runs, including the reference's quirks Q4
+    (right vy ghost column) and Q5 (y1 pressure mirror), with plane velocity
+    and pressure sources on the y0 row."""
+    cfg = cases.sample_like_case(40)
+    cfg.update(grid_y=40, grid_ly=40e-4, pml_x1=0, pml_y1=0, tr_focused_p=0, tr_plane_p=1,
+               tr_plane_v=1, tr_focal_dist=3e-3, tr_box_size=1.0e-3, tr_cylindrical=0)
+    g, o, gs, os_ = run_pair(brp, cfg, steps=120)
+    err = report("mirrors", g, o, FIELDS + ("fg",))
+    for n in FIELDS:
+        assert err[n] < 1e-12, n
+    assert err["fg"] < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cavitating_bubbles(brp, prec):
+    cfg, b = cases.bubbly_case(64, nb=160, precision=prec)
+    g, o, gs, os_ = run_pair(brp, cfg, b, steps=120)
+    err = report(f"bubbly prec={prec}", g, o, FIELDS + ("fg",))
+    rt = rel_l2(g.bubbles("R_t"), o.bubbles("R_t"))
+    print("R_t relL2", rt, "max sub-steps", gs[-1]["rp_max_substeps"], flush=True)
+    assert max(s["rp_max_substeps"] for s in os_) > 100  # inertial collapse is exercised
+    assert [s["n_pic"] for s in gs] == [s["n_pic"] for s in os_]
+    tol = 1e-9 if prec == 0 else 1e-4
+    assert err["p0"] < tol and err["fg"] < tol and rt < tol
+    for n in ("ibn_x", "ibn_y", "ibm_x", "ibm_y"):
+        if prec == 0:
+            np.testing.assert_array_equal(g.bubbles(n), o.bubbles(n))
+    if prec == 0:
+        for n in ("PG", "PL_p", "PL_n", "PL_m", "d1R", "R_p", "R_n", "dt", "re", "Q_B"):
+            a, r = g.bubbles(n), o.bubbles(n)
+            assert rel_l2(a, r) < 1e-8, n
+
+
+def test_binning_bit_exact_at_bin_edges(brp):
+    cfg = cases.abi.water_config(32, npml=4, focal_dist=2e-3)
+    dx = cfg.grid_lx / 32
+    xs = []
+    for k in range(1, 32):
+        for base in (k * dx, (k - 0.5) * dx, (k + 0.5) * dx):
+            for d in range(-3, 4):
+                xs.append(base + d * np.spacing(base))
+    xs = np.array(xs)
+    ys = np.roll(xs, 7)
+    bub = (xs, ys, np.full(len(xs), 2e-6), None, 1e-6)
+    g = cases.make(brp.Simulation, cfg, bub)
+    o = cases.make(ol.OracleSim, cfg, bub)
+    for n in ("ibn_x", "ibn_y", "ibm_x", "ibm_y"):
+        np.testing.assert_array_equal(g.bubbles(n), o.bubbles(n))
+    # the initial void fraction of coincident-edge bubbles
+    assert rel_l2(g.field("fg"), o.field("fg")) < 1e-12
+
+
+def test_source_elements(brp):
+    """Extension E4: 16 aperture elements with their own amplitude and phase."""
+    cfg = cases.acoustic_case(64)
+    r = np.random.default_rng(7)
+    amp = r.uniform(0.3e6, 0.7e6, 16)
+    ph = r.uniform(0, 2 * np.pi, 16)
+    g, o, gs, os_ = run_pair(brp, cfg, steps=120, source=(amp, ph))
+    err = report("elements", g, o, FIELDS)
+    assert err["p0"] < 1e-12
+
+
+def test_deterministic_and_restore_bit_exact(brp):
+    cfg, b = cases.bubbly_case(48, nb=400, seed=3, precision=1)
+    runs = []
+    for _ in range(2):
+        g = cases.make(brp.Simulation, cfg, b)
+        g.step(60)
+        runs.append((g.field("p0"), g.field("fg"), g.bubbles("R_t")))
+        g.close()
+    for a, c in zip(*runs):
+        np.testing.assert_array_equal(a, c)
+    g = cases.make(brp.Simulation, cfg, b)
+    g.step(30)
+    g.save_state(3)
+    g.step(30)
+    first = (g.field("p0"), g.bubbles("R_t"), g.info()["time"])
+    g.restore_state(3)
+    g.step(30)
+    second = (g.field("p0"), g.bubbles("R_t"), g.info()["time"])
+    np.testing.assert_array_equal(first[0], second[0])
+    np.testing.assert_array_equal(first[1], second[1])
+    assert first[2] == second[2]
+    np.testing.assert_array_equal(first[0], runs[0][0])
+
+
+def test_wide_band_and_edge_bubbles(brp):
+    """smooth_delta_band = 4, bubbles straddling the domain edges and outside it."""
+    cfg, _ = cases.bubbly_case(48, nb=10)
+    cfg.sim_smooth_delta_band = 4
+    L = cfg.grid_lx
+    x = np.array([0.3e-4, L - 0.2e-4, L / 2, -1e-3, L / 3, 2.1e-4, L + 1e-3])
+    y = np.array([L / 2, L / 2, 0.4e-4, L / 2, L - 0.3e-4, 1.7e-4, L / 2])
+    bub = (x, y, np.full(len(x), 3e-6), None, 1e-5)
+    g, o, gs, os_ = run_pair(brp, cfg, bub, steps=80)
+    err = report("band4/edges", g, o, FIELDS + ("fg",))
+    assert err["p0"] < 1e-12 and err["fg"] < 1e-12
+
+
+def test_error_paths(brp):
+    cfg, b = cases.bubbly_case(48, nb=50)
+    cfg.max_picard = 1
+    g = cases.make(brp.Simulation, cfg, b)
+    with pytest.raises(brp.BrpError) as e:
+        g.step(100)
+    assert e.value.code == brp.abi.ERR_PICARD_CAP
+    cfg, b = cases.bubbly_case(48, nb=50)
+    cfg.max_substeps = 5
+    g = cases.make(brp.Simulation, cfg, b)
+    with pytest.raises(brp.BrpError) as e:
+        g.step(100)
+    assert e.value.code == brp.abi.ERR_SUBSTEP_CAP
+    bad = cases.acoustic_case(32)
+    bad.sim_order = 4
+    with pytest.raises(brp.BrpError):
+        brp.Simulation(bad)
+
+
+def test_standing_wave_closed_form(brp):
+    """The GPU FDTD reproduces the staggered scheme's exact discrete
+    standing-wave solution (the oracle pin of the same case)."""
+    import test_oracle_pins as pins
+    n, m = 32, 256
+    c = pins.standing_wave_case(n, m, 1)
+    g = brp.Simulation(c)
+    g.prepare()
+    prof = pins.standing_wave_init(g, n, m, 1)
+    g.step(300)
+    amp = pins.standing_wave_exact(c, n, 1, 300)
+    np.testing.assert_allclose(g.field("p0")[10, 1:n + 1], amp * prof[1:n + 1], atol=1e-11)
+
+
+def test_golden_fixture(brp):
+    """The committed 32^2 / 16-bubble / 50-step fixture (tests/golden)."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "c1mini.npz")
+    z = np.load(path)
+    cfg = cases.abi.Config()
+    for k in cases.abi.Config._fields_:
+        if k[0] in z.files:
+            setattr(cfg, k[0], z[k[0]].item())
+    for prec, tol in ((0, 1e-9), (1, 1e-4)):
+        cfg.precision = prec
+        g = cases.make(brp.Simulation, cfg, (z["bx"], z["by"], z["bR0"], None, z["fg_in"].item()))
+        st = g.step(int(z["steps"]))
+        assert [s["n_pic"] for s in st] == list(z["n_pic"])
+        assert rel_l2(g.field("p0"), z["p0"]) < tol
+        assert rel_l2(g.field("fg"), z["fg"]) < tol
+        assert rel_l2(g.bubbles("R_t"), z["R_t"]) < tol
