@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled FDTD + Rayleigh-Plesset step on one MI355X.
+
+Workload (one "step" = one coupled mixture time step of the whole problem):
+  2048 x 2048 cells, dx = dy = 0.1 mm, 32-cell PML on all sides, water
+  (c = 1500 m/s, rho = 1000 kg/m^3, mu = 1e-3 Pa s, sigma = 0.0725 N/m,
+  P0 = 101325 Pa, gamma = 1.4), focused 1 MHz continuous wave of 1.5 MPa from
+  the y0 edge (focal length 15 mm, focus centred in x), 2^20 bubbles with
+  log-normal R0 (median 3 um, sigma_log 0.4) uniform in an 8 x 8 mm box around
+  the focus (seed 2), initial mean void fraction 1e-6 in the box, CFL 0.5.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+W untimed steps bring the wavefront to the bubbles (the focus is ~300 steps
+from the source), then K steps are timed with the mixed-precision path
+(FP32 fields, FP64 RP and void fraction), then the same W + K steps run with
+the all-FP64 path for the speed and error comparison.  One JSON line on
+stdout.  Progress lines go to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "cell-updates/s & bubble-steps/s: mixed vs FP64 at stated err; adjoint/fwd time"
+# Algorithmic FP64 operations of one RP sub-step (add, sub, mul, div, sqrt,
+# tanh, tan each counted once; the oracle's operation counter gives 470.6 on a
+# driven log-normal population) and of one bubble's gather + scatter.
+RP_FLOPS_PER_SUBSTEP = 470.0
+BUBBLE_FLOPS_PER_CALL = 60.0
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c2_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
+    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=1.5e6, freq=1.0e6, focal_dist=15e-3,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = np.random.default_rng(seed)
+    fpx, fpy = cfg.ext_focus_x, 15e-3 * 0.5 * math.sqrt(3.0)
+    box = 8e-3
+    x = fpx + (r.random(nb) - 0.5) * box
+    y = fpy + (r.random(nb) - 0.5) * box
+    R0 = np.exp(math.log(3e-6) + 0.4 * r.standard_normal(nb))
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    return cfg, (x, y, R0, None, fg_in)
+
+
+def stepped(sim, k, tag, chunk=25):
+    """sim.step(k) in chunks, printing one progress line per chunk."""
+    out = []
+    phases = None
+    done = 0
+    while done < k:
+        m = min(chunk, k - done)
+        t0 = time.perf_counter()
+        st = sim.step(m)
+        dt = time.perf_counter() - t0
+        ph = sim.phase_times()
+        phases = ph if phases is None else {a: phases[a] + ph[a] for a in ph}
+        done += m
+        out += st
+        log(f"  {tag} step {st[-1]['nstep']}: {dt / m * 1e3:.2f} ms/step, Picard "
+            f"{sum(s['n_pic'] for s in st) / m:.2f}, RP sub-steps mean "
+            f"{np.mean([s['rp_mean_substeps'] for s in st]):.1f} max "
+            f"{max(s['rp_max_substeps'] for s in st)}")
+    return out, phases
+
+
+def run_gpu(brp, cfg, bub, warmup, steps, first):
+    sim = brp.Simulation(cfg)
+    sim.set_bubbles(*bub)
+    sim.prepare()
+    t0 = time.perf_counter()
+    st_first = sim.step(first) if first else []
+    t_first = time.perf_counter() - t0
+    if warmup > first:
+        stepped(sim, warmup - first, "warmup", chunk=50)
+    log(f"  warmup done ({warmup} steps), timing {steps} steps")
+    t0 = time.perf_counter()
+    st, phases = stepped(sim, steps, "timed")
+    wall = time.perf_counter() - t0
+    return sim, st, wall, phases, t_first, st_first
+
+
+def cpu_baseline(abi, cfg, bub, sample_steps, threads):
+    import oracle_lib as ol
+    o = ol.OracleSim(cfg, threads=threads)
+    o.set_bubbles(*bub)
+    o.prepare()
+    t0 = time.perf_counter()
+    st = o.step(sample_steps)
+    wall = time.perf_counter() - t0
+    o.close()
+    return wall, st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--bubbles", type=int, default=1 << 20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the coupled step runs on one MI355X; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import importlib
+    brp = importlib.import_module("cuda-rp_amd")
+    abi = brp.abi
+    n, nb = args.grid, args.bubbles
+    first = min(args.cpu_steps, args.warmup)
+
+    cfg_m, bub = c2_workload(abi, abi.PREC_MIXED, n, nb)
+    log(f"mixed: {n}^2 cells, {nb} bubbles, warmup {args.warmup}, steps {args.steps}")
+    sim_m, st_m, wall_m, ph_m, tfirst_m, _ = run_gpu(brp, cfg_m, bub, args.warmup, args.steps,
+                                                     first)
+    cells = n * n
+    cups = cells * args.steps / wall_m
+    bsps = nb * args.steps / wall_m
+    launches = sum(s["n_pic"] for s in st_m)
+    substeps = sum(s["rp_mean_substeps"] * s["n_pic"] * nb for s in st_m)
+    rp_ms = ph_m["bubble_rp"]
+    avg_ms = rp_ms / max(launches, 1)
+    flops_per_launch = (RP_FLOPS_PER_SUBSTEP * substeps + BUBBLE_FLOPS_PER_CALL * nb * launches) \
+        / max(launches, 1)
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    log(f"  mixed: {wall_m / args.steps * 1e3:.3f} ms/step, {cups:.3e} cell-updates/s, "
+        f"phases {ph_m}, Picard {launches / args.steps:.2f}/step")
+
+    out = {
+        "metric": METRIC,
+        "value": cups,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_m / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields + f64 RP / int64 void fraction (mixed)",
+        "data": "synthetic (seeded bubble population, analytic source)",
+        "config": {"workload": f"{n}x{n} grid, 32-cell PML, 1 MHz 1.5 MPa focused CW, "
+                               f"{nb} log-normal bubbles in an 8x8 mm box, CFL 0.5",
+                   "grid": n, "bubbles": nb, "precision": "mixed",
+                   "steps_window": f"{args.warmup + 1}-{args.warmup + args.steps}"},
+        "bubble_steps_per_s": bsps,
+        "picard_per_step": launches / args.steps,
+        "rp_substeps_per_s": substeps / wall_m,
+        "phase_ms_per_step": {k: v / args.steps for k, v in ph_m.items()},
+        "roofline": {"kernel": "k_bubble_step (gather + Rayleigh-Plesset + scatter)",
+                     "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "flops_per_launch": flops_per_launch,
+                     "note": "FP64 vector ALU bound; MI355X FP64 vector and matrix peaks "
+                             "are both 78.6 TFLOP/s"},
+    }
+
+    if not args.no_fp64:
+        cfg_d, _ = c2_workload(abi, abi.PREC_FP64, n, nb)
+        log("fp64 run")
+        sim_d, st_d, wall_d, ph_d, _, _ = run_gpu(brp, cfg_d, bub, args.warmup, args.steps, 0)
+        cups_d = cells * args.steps / wall_d
+        from cases import rel_l2
+        err_p0 = rel_l2(sim_m.field("p0"), sim_d.field("p0"))
+        err_rt = rel_l2(sim_m.bubbles("R_t"), sim_d.bubbles("R_t"))
+        out["fp64"] = {"value": cups_d, "ms_per_step": wall_d / args.steps * 1e3,
+                       "bubble_steps_per_s": nb * args.steps / wall_d,
+                       "phase_ms_per_step": {k: v / args.steps for k, v in ph_d.items()},
+                       "picard_per_step": sum(s["n_pic"] for s in st_d) / args.steps}
+        out["mixed_vs_fp64"] = {"speedup": cups / cups_d, "p0_rel_l2": err_p0,
+                                "R_t_rel_l2": err_rt,
+                                "picard_count_mismatch_steps":
+                                    int(sum(a["n_pic"] != b["n_pic"] for a, b in zip(st_m, st_d)))}
+        log(f"  fp64: {wall_d / args.steps * 1e3:.3f} ms/step; p0 err {err_p0:.2e}")
+        sim_d.close()
+    sim_m.close()
+
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cfg_c, _ = c2_workload(abi, abi.PREC_FP64, n, nb)
+        log(f"cpu baseline: oracle, {threads} threads, steps 1-{first}")
+        wall_c, st_c = cpu_baseline(abi, cfg_c, bub, first, threads)
+        out["cpu_baseline"] = {
+            "value": cells * first / wall_c, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"the FP64 oracle on this workload, steps 1-{first} from rest "
+                      f"({wall_c:.1f} s; the bubbles are still at equilibrium)",
+            "bubble_steps_per_s": nb * first / wall_c,
+            "gpu_same_sample_value": cells * first / tfirst_m,
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -439,7 +439,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
                 loop++;
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
                 launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
-                                      p0, h->fgb[target], h->sc_d, 1);
+                                      p0, h->fgb[target], h->sc_d, loop);
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
                 HIPCHK(h, hipEventRecord(h->ev[5], s));
                 launch_pressure_picard<F>(s, h->g, vx, vy, (F *)h->pb[pn_i][0], (F *)h->pb[pn_i][1],

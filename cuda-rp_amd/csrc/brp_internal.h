@@ -63,6 +63,7 @@ struct StepScalars {
     int sub_max;                         // max RP sub-steps this step
     unsigned long long sub_sum;          // sum of RP sub-steps this step
     unsigned long long resid[kMaxEval];  // max |dp0| bit patterns (>= 0 doubles)
+    unsigned int work[kMaxEval];         // bubble work-queue heads, one per Picard iteration
 };
 
 // Bubble arrays (device pointers; n padded to a multiple of 256).
@@ -118,7 +119,8 @@ void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
 template <typename F>
 void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                         const BubbleArrays &b, int cur, int plm, int pln, int plp,
-                        const F *p0, int64_t *fg_target, StepScalars *sc, int do_rp);
+                        const F *p0, int64_t *fg_target, StepScalars *sc, int slot);
+int bubble_step_blocks(int n_bubbles);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);

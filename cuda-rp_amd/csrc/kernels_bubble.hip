@@ -10,6 +10,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace brp {
 namespace {
 
@@ -160,88 +162,162 @@ __device__ inline double fg_temp_of(const GridParams &g, const BubbleParams &bp,
     return v;
 }
 
+// Gather of p0 at a bubble (BubbleInterpolationScalarKernel,
+// bubble_CUDA_kernel.cuh:361-395) and the bubble's B x B stencil weights.
+template <typename F>
+__device__ inline double gather_p0(const GridParams &g, const BubbleParams &bp, const F *p0,
+                                   double x, double y, int ibnx, int ibny) {
+    const int nb = bp.mbe - bp.mbs + 1;
+    double PL = 0.0;
+    for (int a = 0; a < nb; a++) {
+        const int i = ibnx + bp.mbs + a;
+        const double Dx = delta_w(bp.dcx, bp.dcy, i, g.dx, x);
+        for (int c = 0; c < nb; c++) {
+            const int j = ibny + bp.mbs + c;
+            const double Dy = delta_w(bp.dcx, bp.dcz, j, g.dy, y);
+            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+#pragma clang fp contract(off)
+                PL += (double)p0[frame_off(i, j, g.pitch)] * Dx * Dy;
+            }
+        }
+    }
+    return PL;
+}
+
+__device__ inline void scatter_bubble(const GridParams &g, const BubbleParams &bp, double x,
+                                      double y, int ibnx, int ibny, double fg_temp, int64_t *fg,
+                                      bool *bad) {
+    const int nb = bp.mbe - bp.mbs + 1;
+    double wx[kMaxBand], wy[kMaxBand];
+    for (int a = 0; a < nb; a++) {
+        wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
+        wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
+    }
+    scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp, fg, bad);
+}
+
+__device__ inline unsigned long long lanemask_lt(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
 // One Picard iteration for every bubble, fused:
 //   BubbleInterpolationScalarKernel (gather PL_p, kernel.cuh:361-395)
 //   BubbleRadiusKernel              (RP from the step's start state, 471-554)
 //   VoidFractionReverseLookupKernel (scatter R_t into fg_target, 576-619)
+//
+// Persistent kernel with lane-level work refill.  The number of RP sub-steps
+// differs by orders of magnitude between bubbles (about 11 at rest, thousands
+// through an inertial collapse), so one bubble per lane would run each
+// wavefront at the pace of its slowest lane.  Here every lane integrates one
+// sub-step per loop pass; a lane whose bubble reaches the sync time writes it
+// back and takes the next bubble at once.  Bubbles are handed to wavefronts in
+// batches of 64 consecutive (cell-sorted) indices through one atomic counter
+// per Picard iteration, so a wavefront costs the sum, not the max, of its
+// lanes' sub-steps.  Per bubble the arithmetic is exactly that of the
+// reference kernel, so results do not depend on the schedule.
 template <typename F>
-__global__ __launch_bounds__(256) void k_bubble_step(GridParams g, BubbleParams bp,
-                                                      BubbleArrays b, int cur, int plm, int pln,
-                                                      int plp, const F *__restrict__ p0,
-                                                      int64_t *__restrict__ fg_target,
-                                                      StepScalars *sc) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    int sub = 0;
-    bool bad = false, capped = false;
-    if (idx < b.n) {
-        const double x = b.x[idx], y = b.y[idx];
-        const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
-        const int nb = bp.mbe - bp.mbs + 1;
-        double wx[kMaxBand], wy[kMaxBand];
-        for (int a = 0; a < nb; a++) {
-            wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
-            wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
-        }
-        double PL = 0.0;
-        for (int a = 0; a < nb; a++) {
-            const int i = ibnx + bp.mbs + a;
-            for (int c = 0; c < nb; c++) {
-                const int j = ibny + bp.mbs + c;
-                if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
-#pragma clang fp contract(off)
-                    PL += (double)p0[frame_off(i, j, g.pitch)] * wx[a] * wy[c];
+__global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleParams bp,
+                                                            BubbleArrays b, int cur, int plm,
+                                                            int pln, int plp,
+                                                            const F *__restrict__ p0,
+                                                            int64_t *__restrict__ fg_target,
+                                                            StepScalars *sc, int slot) {
+    const int lane = threadIdx.x & 63;
+    const int nxt = cur ^ 1;
+    const RPShared sh{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
+    int idx = -1;                  // bubble of this lane, -1 when idle
+    int batch_next = 0, batch_end = 0;  // wave-uniform
+    bool exhausted = false;             // wave-uniform
+    RPLane L;
+    RPConst k{0.0, 0.0, 0.0, 0.0};
+    double x = 0.0, y = 0.0;
+    int ibnx = 0, ibny = 0;
+    int sub_max = 0;
+    unsigned long long sub_sum = 0;
+    bool bad = false, capped_any = false;
+    while (true) {
+        // ---- refill idle lanes from the wave's batch / the global queue
+        bool need = idx < 0;
+        unsigned long long m = __ballot(need);
+        bool fresh = false;
+        while (m != 0ull && !exhausted) {
+            if (batch_next >= batch_end) {
+                const int leader = __ffsll((long long)m) - 1;
+                unsigned int base = 0;
+                if (lane == leader) base = atomicAdd(&sc->work[slot], 64u);
+                base = __shfl(base, leader, 64);
+                if ((int)base >= b.n) {
+                    exhausted = true;
+                    break;
                 }
+                batch_next = (int)base;
+                batch_end = min((int)base + 64, b.n);
+            }
+            const int avail = batch_end - batch_next;
+            const int rank = __popcll(m & lanemask_lt(lane));
+            if (need && rank < avail) {
+                idx = batch_next + rank;
+                need = false;
+                fresh = true;
+            }
+            batch_next += min(avail, __popcll(m));
+            m = __ballot(need);
+        }
+        if (fresh) {
+            x = b.x[idx];
+            y = b.y[idx];
+            ibnx = bin_index(x, g.rdx, 0.5);
+            ibny = bin_index(y, g.rdy, 0.5);
+            const double PL = gather_p0(g, bp, p0, x, y, ibnx, ibny);
+            b.PL[plp][idx] = PL;
+            const double R0 = b.R0[idx];
+            k = RPConst{R0, R0 * R0 * R0, b.PG0[idx], b.calpha[idx]};
+            RPState in{b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx],
+                       b.PG[cur][idx], b.bdt[cur][idx], b.re[cur][idx]};
+            rp_begin(L, in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, sh);
+        }
+        const bool active = idx >= 0;
+        if (__ballot(active) == 0ull) break;
+        // ---- one sub-step, or write back a bubble that reached the sync time
+        if (active) {
+            const bool capped = L.counter >= bp.max_substeps;
+            if (L.remain > 0.0 && !capped) {
+                rp_substep(L, k, sh);
+            }
+            if (!(L.remain > 0.0) || capped) {
+                RPState out;
+                double R_t, Q_B;
+                rp_end(L, g.dt, &out, &R_t, &Q_B);
+                b.R[nxt][idx] = out.Rp;
+                b.Rn[nxt][idx] = out.Rn;
+                b.d1R[nxt][idx] = out.d1R;
+                b.PG[nxt][idx] = out.PG;
+                b.bdt[nxt][idx] = out.dt;
+                b.re[nxt][idx] = out.re;
+                b.R_t[idx] = R_t;
+                b.Q_B[idx] = Q_B;
+                b.sub[idx] = L.counter;
+                sub_max = max(sub_max, L.counter);
+                sub_sum += (unsigned long long)L.counter;
+                capped_any |= capped && L.remain > 0.0;
+                bad |= !isfinite(R_t);
+                scatter_bubble(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], R_t, x),
+                               fg_target, &bad);
+                idx = -1;
             }
         }
-        b.PL[plp][idx] = PL;
-        const int nxt = cur ^ 1;
-        RPState in{b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx],
-                   b.PG[cur][idx], b.bdt[cur][idx], b.re[cur][idx]};
-        RPConst k{b.R0[idx], b.R0[idx] * b.R0[idx] * b.R0[idx], b.PG0[idx], b.calpha[idx]};
-        RPShared s{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
-        RPState out;
-        double R_t, Q_B;
-        sub = rp_step(in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, k, s, bp.max_substeps,
-                      &out, &R_t, &Q_B, &capped);
-        b.R[nxt][idx] = out.Rp;
-        b.Rn[nxt][idx] = out.Rn;
-        b.d1R[nxt][idx] = out.d1R;
-        b.PG[nxt][idx] = out.PG;
-        b.bdt[nxt][idx] = out.dt;
-        b.re[nxt][idx] = out.re;
-        b.R_t[idx] = R_t;
-        b.Q_B[idx] = Q_B;
-        b.sub[idx] = sub;
-        bad |= !isfinite(R_t);
-        scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp_of(g, bp, b.n_B[idx], R_t, x), fg_target,
-                    &bad);
     }
-    // per-step statistics: max and sum of sub-steps, error flags
-    int smax = sub;
-    unsigned long long ssum = (unsigned long long)sub;
     for (int off = 32; off > 0; off >>= 1) {
-        smax = max(smax, __shfl_xor(smax, off, 64));
-        ssum += __shfl_xor(ssum, off, 64);
+        sub_max = max(sub_max, __shfl_xor(sub_max, off, 64));
+        sub_sum += __shfl_xor(sub_sum, off, 64);
     }
-    __shared__ int rmax[4];
-    __shared__ unsigned long long rsum[4];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        rmax[w] = smax;
-        rsum[w] = ssum;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k2 = 1; k2 < (int)(blockDim.x >> 6); k2++) {
-            smax = max(smax, rmax[k2]);
-            ssum += rsum[k2];
-        }
-        if (ssum) atomicAdd(&sc->sub_sum, ssum);
-        if (smax > __hip_atomic_load(&sc->sub_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&sc->sub_max, smax);
+    if (lane == 0) {
+        if (sub_sum) atomicAdd(&sc->sub_sum, sub_sum);
+        if (sub_max > __hip_atomic_load(&sc->sub_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&sc->sub_max, sub_max);
     }
     if (bad) atomicOr(&sc->flags, 2);
-    if (capped) atomicOr(&sc->flags, 4);
+    if (capped_any) atomicOr(&sc->flags, 4);
 }
 
 // The initial void fraction (calculate_void_fraction at set-up,
@@ -301,13 +377,26 @@ void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                        sc);
 }
 
+int bubble_step_blocks(int n_bubbles) {
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bubble_persistent<float>,
+                                                           256, 0);
+        cached = std::max(1, cus) * std::max(1, per_cu);
+    }
+    return std::max(1, std::min(cached, (n_bubbles + 255) / 256));
+}
+
 template <typename F>
 void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                         const BubbleArrays &b, int cur, int plm, int pln, int plp, const F *p0,
-                        int64_t *fg_target, StepScalars *sc, int) {
+                        int64_t *fg_target, StepScalars *sc, int slot) {
     if (b.n <= 0) return;
-    hipLaunchKernelGGL(k_bubble_step<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, cur,
-                       plm, pln, plp, p0, fg_target, sc);
+    hipLaunchKernelGGL(k_bubble_persistent<F>, dim3(bubble_step_blocks(b.n)), dim3(256), 0, s,
+                       g, bp, b, cur, plm, pln, plp, p0, fg_target, sc, slot);
 }
 
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,

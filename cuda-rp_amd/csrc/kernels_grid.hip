@@ -332,7 +332,10 @@ __global__ __launch_bounds__(1024) void k_pressure_ghosts(GridParams g, F *p0,
 
 __global__ void k_zero_scalars(StepScalars *sc, int nslots) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nslots) sc->resid[t] = 0ull;
+    if (t < nslots) {
+        sc->resid[t] = 0ull;
+        sc->work[t] = 0u;
+    }
     if (t == 0) {
         sc->bbox[0] = 0x7fffffff;
         sc->bbox[1] = -0x7fffffff;

@@ -149,52 +149,66 @@ struct RPState {
     double Rp, Rn, d1R, PG, dt, re;
 };
 
-// One mixture step of one bubble (the body of BubbleRadiusKernel): integrates
-// from the step's start state `in` with the liquid pressure history
-// (PL_m, PL_n, PL_p), writes the end state to `out`, R_t and Q_B, and returns
-// the sub-step count (capped at max_sub; *capped set when the cap stops it).
-__device__ inline int rp_step(const RPState &in, double PLm, double PLn, double PLp,
-                              double dt_mix, const RPConst &k, const RPShared &s, int max_sub,
-                              RPState *out, double *R_t, double *Q_B, bool *capped) {
-    double Rp = in.Rp, Rn = in.Rn, d1Rp = in.d1R, PGp = in.PG, dt_L = in.dt;
-    double remain = in.re + dt_mix;
-    double time = -in.re;
-    double PC0 = PLn + s.PL0;
-    double PC1 = 0.5 * (PLp - PLm) / dt_mix;
-    double PC2 = 0.5 * (PLp + PLm - 2.0 * PLn) / (dt_mix * dt_mix);
-    double SumHeat = 0.0, SumVis = 0.0, Rt = 0.0;
-    int counter = 0;
-    *capped = false;
-    while (remain > 0.0) {
-        if (counter >= max_sub) {
-            *capped = true;
-            break;
-        }
-        double PGn = PGp;
-        double PL = PC2 * time * time + PC1 * time + PC0;
-        rp_radius(&Rt, &Rp, &Rn, &d1Rp, PGn, PL, &dt_L, &remain, s);
-        time = time + dt_L;
-        cd alpha;
-        double om = rp_omega(&alpha, PGn, Rn, k, s);
-        cd Lp = rp_lp(alpha, Rn);
-        PGp = rp_pg(PGn, Rp, Rn, om, dt_L, Lp, k, s);
-        double s0 = 0.5 * (Rp + Rn);
-        double s1 = 1 / (Lp.re * Lp.re + Lp.im * Lp.im) * s.T0 / (k.PG0 * k.R03);
-        double dTdr = Lp.re * s1 * (k.PG0 * k.R03 - 0.5 * (PGp + PGn) * s0 * s0 * s0) +
-                      Lp.im * s1 * (PGp * Rp * Rp * Rp - PGn * Rn * Rn * Rn) / (om * dt_L);
-        SumHeat -= 4.0 * s.pi * Rp * Rp * s.K0 * dTdr * dt_L;
-        SumVis += 4.0 * s.pi * Rp * Rp * 4.0 * s.mu * d1Rp / Rp * d1Rp * dt_L;
-        counter++;
-    }
-    out->Rp = Rp;
-    out->Rn = Rn;
-    out->d1R = d1Rp;
-    out->PG = PGp;
-    out->dt = dt_L;
-    out->re = remain;
-    *R_t = Rt;
-    *Q_B = (SumHeat + SumVis) / (dt_mix - remain + in.re);
-    return counter;
+// The integration state of one lane between sub-steps.
+struct RPLane {
+    double Rp, Rn, d1R, PG, dtL, remain, time, PC0, PC1, PC2, SumHeat, SumVis, Rt, re_n;
+    int counter;
+};
+
+// Entry of BubbleRadiusKernel (bubble_CUDA_kernel.cuh:487-504): start state,
+// remaining time and the quadratic liquid-pressure history through
+// (PL_m, PL_n, PL_p).
+__device__ inline void rp_begin(RPLane &L, const RPState &in, double PLm, double PLn, double PLp,
+                                double dt_mix, const RPShared &s) {
+    L.Rp = in.Rp;
+    L.Rn = in.Rn;
+    L.d1R = in.d1R;
+    L.PG = in.PG;
+    L.dtL = in.dt;
+    L.re_n = in.re;
+    L.remain = in.re + dt_mix;
+    L.time = -in.re;
+    L.PC0 = PLn + s.PL0;
+    L.PC1 = 0.5 * (PLp - PLm) / dt_mix;
+    L.PC2 = 0.5 * (PLp + PLm - 2.0 * PLn) / (dt_mix * dt_mix);
+    L.SumHeat = 0.0;
+    L.SumVis = 0.0;
+    L.Rt = 0.0;
+    L.counter = 0;
+}
+
+// One pass of the sub-step loop body (bubble_CUDA_kernel.cuh:508-538).
+__device__ inline void rp_substep(RPLane &L, const RPConst &k, const RPShared &s) {
+    double PGn = L.PG;
+    double PL = L.PC2 * L.time * L.time + L.PC1 * L.time + L.PC0;
+    rp_radius(&L.Rt, &L.Rp, &L.Rn, &L.d1R, PGn, PL, &L.dtL, &L.remain, s);
+    L.time = L.time + L.dtL;
+    cd alpha;
+    double om = rp_omega(&alpha, PGn, L.Rn, k, s);
+    cd Lp = rp_lp(alpha, L.Rn);
+    double PGp = rp_pg(PGn, L.Rp, L.Rn, om, L.dtL, Lp, k, s);
+    L.PG = PGp;
+    double Rp = L.Rp, Rn = L.Rn;
+    double s0 = 0.5 * (Rp + Rn);
+    double s1 = 1 / (Lp.re * Lp.re + Lp.im * Lp.im) * s.T0 / (k.PG0 * k.R03);
+    double dTdr = Lp.re * s1 * (k.PG0 * k.R03 - 0.5 * (PGp + PGn) * s0 * s0 * s0) +
+                  Lp.im * s1 * (PGp * Rp * Rp * Rp - PGn * Rn * Rn * Rn) / (om * L.dtL);
+    L.SumHeat -= 4.0 * s.pi * Rp * Rp * s.K0 * dTdr * L.dtL;
+    L.SumVis += 4.0 * s.pi * Rp * Rp * 4.0 * s.mu * L.d1R / Rp * L.d1R * L.dtL;
+    L.counter++;
+}
+
+// Exit of BubbleRadiusKernel (bubble_CUDA_kernel.cuh:542-550).
+__device__ inline void rp_end(const RPLane &L, double dt_mix, RPState *out, double *R_t,
+                              double *Q_B) {
+    out->Rp = L.Rp;
+    out->Rn = L.Rn;
+    out->d1R = L.d1R;
+    out->PG = L.PG;
+    out->dt = L.dtL;
+    out->re = L.remain;
+    *R_t = L.Rt;
+    *Q_B = (L.SumHeat + L.SumVis) / (dt_mix - L.remain + L.re_n);
 }
 
 }  // namespace brp
