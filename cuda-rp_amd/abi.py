@@ -57,7 +57,7 @@ class Config(C.Structure):
         ("ext_focus_x", C.c_double), ("ext_focus_x_set", C.c_int32),
         ("ext_source_in_pml", C.c_int32), ("precision", C.c_int32),
         ("max_picard", C.c_int32), ("max_substeps", C.c_int32), ("thermal", C.c_int32),
-        ("reserved0", C.c_int32), ("pad1_", C.c_int32),
+        ("rp_exact", C.c_int32), ("pad1_", C.c_int32),
     ]
 
     def update(self, **kw):

@@ -263,6 +263,8 @@ int alloc_bubbles(brp_handle *h, int n) {
     for (double **a : arrs)
         if ((rc = dalloc(h, (void **)a, bytes))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.sub, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->b.order, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->b.part, sizeof(int) * (size_t)((n + 255) / 256 + 2)))) return rc;
     return BRP_OK;
 }
 
@@ -438,6 +440,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
                 }
                 loop++;
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
+                launch_order_partition(s, h->b, h->bp.heavy_threshold);
                 launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
                                       p0, h->fgb[target], h->sc_d, loop);
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
@@ -565,7 +568,6 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     if (c.precision != BRP_PREC_FP64 && c.precision != BRP_PREC_MIXED)
         return set_err(nullptr, BRP_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
     if (c.thermal) return set_err(nullptr, BRP_ERR_INVALID, "thermal solve is not available in this build");
-    if (c.reserved0) return set_err(nullptr, BRP_ERR_INVALID, "reserved0 must be 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return set_err(nullptr, BRP_ERR_NO_DEVICE, "no HIP device visible");
@@ -668,6 +670,8 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     bp.pi4r3 = bp.pi * 4.0 / 3.0;
     bp.max_substeps = c.max_substeps > 0 ? c.max_substeps : 1000000;
     bp.cylindrical = c.tr_cylindrical;
+    bp.rp_exact = c.rp_exact ? 1 : 0;
+    bp.heavy_threshold = 48;
 
     int rc = h->prec == BRP_PREC_FP64 ? alloc_grid<double>(h) : alloc_grid<float>(h);
     if (rc) return fail(rc);

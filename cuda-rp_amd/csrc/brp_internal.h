@@ -76,6 +76,8 @@ struct BubbleArrays {
     double *n_B, *R0, *PG0, *calpha;
     double *vx, *vy;      // v_B (= v_L)
     int *sub;             // sub-steps of the last RP call, per bubble
+    int *order;           // work-queue order: bubbles with many sub-steps first
+    int *part;            // partition scratch: per-block counts, then the heavy total
 };
 
 // Bubble physics constants shared by every bubble.
@@ -86,6 +88,8 @@ struct BubbleParams {
     double pi4r3, pi;
     int max_substeps;
     int cylindrical;
+    int rp_exact;
+    int heavy_threshold;    // sub-steps that put a bubble at the queue front
 };
 
 }  // namespace brp
@@ -121,6 +125,7 @@ void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &
                         const BubbleArrays &b, int cur, int plm, int pln, int plp,
                         const F *p0, int64_t *fg_target, StepScalars *sc, int slot);
 int bubble_step_blocks(int n_bubbles);
+void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);

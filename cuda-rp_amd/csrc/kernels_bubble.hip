@@ -215,7 +215,7 @@ __device__ inline unsigned long long lanemask_lt(int lane) {
 // per Picard iteration, so a wavefront costs the sum, not the max, of its
 // lanes' sub-steps.  Per bubble the arithmetic is exactly that of the
 // reference kernel, so results do not depend on the schedule.
-template <typename F>
+template <typename F, bool FAST>
 __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleParams bp,
                                                             BubbleArrays b, int cur, int plm,
                                                             int pln, int plp,
@@ -225,6 +225,22 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
     const RPShared sh{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
+    const RPFastShared fs{bp.PL0,
+                          bp.gam,
+                          bp.mu,
+                          bp.rho,
+                          1.0 / bp.rho,
+                          bp.dt0,
+                          3.0 * bp.gam,
+                          3.0 * (bp.gam - 1.0),
+                          3.0 * (bp.gam - 1.0) * bp.K0 * bp.T0,
+                          2.0 * bp.sig,
+                          4.0 * bp.mu,
+                          bp.mu * bp.mu,
+                          -2.0 * bp.sig * bp.rho,
+                          4.0 * bp.pi * bp.K0,
+                          16.0 * bp.pi * bp.mu};
+    RPFastConst fk{0, 0, 0, 0, 0, 0, 0};
     int idx = -1;                  // bubble of this lane, -1 when idle
     int batch_next = 0, batch_end = 0;  // wave-uniform
     bool exhausted = false;             // wave-uniform
@@ -256,7 +272,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
             const int avail = batch_end - batch_next;
             const int rank = __popcll(m & lanemask_lt(lane));
             if (need && rank < avail) {
-                idx = batch_next + rank;
+                idx = b.order[batch_next + rank];
                 need = false;
                 fresh = true;
             }
@@ -272,6 +288,9 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
             b.PL[plp][idx] = PL;
             const double R0 = b.R0[idx];
             k = RPConst{R0, R0 * R0 * R0, b.PG0[idx], b.calpha[idx]};
+            if (FAST)
+                fk = RPFastConst{R0,        1.0 / R0,          k.PG0, 1.0 / k.PG0, k.calpha,
+                                 k.PG0 * k.R03, bp.T0 / (k.PG0 * k.R03)};
             RPState in{b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx],
                        b.PG[cur][idx], b.bdt[cur][idx], b.re[cur][idx]};
             rp_begin(L, in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, sh);
@@ -282,7 +301,10 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
         if (active) {
             const bool capped = L.counter >= bp.max_substeps;
             if (L.remain > 0.0 && !capped) {
-                rp_substep(L, k, sh);
+                if (FAST)
+                    rp_substep_fast(L, fk, fs);
+                else
+                    rp_substep(L, k, sh);
             }
             if (!(L.remain > 0.0) || capped) {
                 RPState out;
@@ -383,7 +405,8 @@ int bubble_step_blocks(int n_bubbles) {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bubble_persistent<float>,
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                           k_bubble_persistent<float, true>,
                                                            256, 0);
         cached = std::max(1, cus) * std::max(1, per_cu);
     }
@@ -395,8 +418,90 @@ void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &
                         const BubbleArrays &b, int cur, int plm, int pln, int plp, const F *p0,
                         int64_t *fg_target, StepScalars *sc, int slot) {
     if (b.n <= 0) return;
-    hipLaunchKernelGGL(k_bubble_persistent<F>, dim3(bubble_step_blocks(b.n)), dim3(256), 0, s,
-                       g, bp, b, cur, plm, pln, plp, p0, fg_target, sc, slot);
+    if (bp.rp_exact)
+        hipLaunchKernelGGL((k_bubble_persistent<F, false>), dim3(bubble_step_blocks(b.n)),
+                           dim3(256), 0, s, g, bp, b, cur, plm, pln, plp, p0, fg_target, sc, slot);
+    else
+        hipLaunchKernelGGL((k_bubble_persistent<F, true>), dim3(bubble_step_blocks(b.n)),
+                           dim3(256), 0, s, g, bp, b, cur, plm, pln, plp, p0, fg_target, sc, slot);
+}
+
+// Stable partition of the work queue: bubbles whose last RP call took at
+// least `threshold` sub-steps first (longest-first scheduling: their long
+// sequential sub-step chains start at once and overlap everything else),
+// each group in index (cell-sorted) order.  Three passes: per-block counts,
+// one-block exclusive scan, scatter.
+__global__ __launch_bounds__(256) void k_part_count(const int *sub, int n, int thr, int *cnt) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const int h = (idx < n && sub[idx] >= thr) ? 1 : 0;
+    const int c = __syncthreads_count(h);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_part_scan(int *cnt, int nblk) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int base = 0; base < nblk; base += 1024) {
+        const int k = base + threadIdx.x;
+        const int v = k < nblk ? cnt[k] : 0;
+        int x = v;  // inclusive wave scan
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int q = 0; q < 16; q++) {
+                const int t = wsum[q];
+                wsum[q] = acc;
+                acc += t;
+            }
+        }
+        __syncthreads();
+        const int excl = carry + wsum[w] + x - v;
+        if (k < nblk) cnt[k] = excl;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[nblk] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_part_scatter(const int *sub, int n, int thr,
+                                                       const int *off, int nblk, int *order) {
+    __shared__ int wcnt[4];
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = idx < n;
+    const bool h = valid && sub[idx] >= thr;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(h);
+    if (lane == 0) wcnt[w] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    for (int q = 0; q < w; q++) before += wcnt[q];
+    const int hr = before + __popcll(m & lanemask_lt(lane));  // heavy rank in block
+    const int total_heavy = off[nblk];
+    const int blk_heavy = off[blockIdx.x];
+    if (valid) {
+        if (h)
+            order[blk_heavy + hr] = idx;
+        else
+            order[total_heavy + (blockIdx.x * 256 - blk_heavy) + (threadIdx.x - hr)] = idx;
+    }
+}
+
+void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold) {
+    if (b.n <= 0) return;
+    const int nblk = (b.n + 255) / 256;
+    hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, b.sub, b.n, threshold, b.part);
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, b.part, nblk);
+    hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, b.sub, b.n, threshold, b.part,
+                       nblk, b.order);
 }
 
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,

@@ -29,11 +29,24 @@ def run_pair(brp, cfg, bubbles=None, steps=100, source=None):
     return g, o, gs, os_
 
 
+# Split pressures and velocity components are normalised by the scale of the
+# whole pressure / velocity field: in a plane wave p.x and v.x are ~1e-12 Pa
+# of noise, and the int64 void fraction rounds the oracle's ~1e-23 FP64 jitter
+# of f_g to exactly zero, so their own relative L2 would measure noise.
+SCALE = {"px": "p0", "py": "p0", "vx": ("vx", "vy"), "vy": ("vx", "vy")}
+
+
 def report(tag, g, o, names):
     out = {}
     for n in names:
         a, b = g.field(n), o.field(n)
-        out[n] = rel_l2(a, b)
+        if n in SCALE:
+            ref = SCALE[n]
+            ref = ref if isinstance(ref, tuple) else (ref,)
+            scale = max(np.linalg.norm(o.field(r)) for r in ref)
+            out[n] = np.linalg.norm(a - b) / scale if scale > 0 else np.linalg.norm(a - b)
+        else:
+            out[n] = rel_l2(a, b)
     print(tag, {k: f"{v:.2e}" for k, v in out.items()}, flush=True)
     return out
 
@@ -81,9 +94,12 @@ def test_all_mirror_walls_and_plane_sources(brp):
     assert err["fg"] < 1e-10
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_cavitating_bubbles(brp, prec):
+@pytest.mark.parametrize("prec,exact", [(0, 1), (0, 0), (1, 0)])
+def test_cavitating_bubbles(brp, prec, exact):
+    """rp_exact = 1 integrates RP in the reference's operation order; 0 (the
+    default) uses the shortened dependency chain, equal up to rounding."""
     cfg, b = cases.bubbly_case(64, nb=160, precision=prec)
+    cfg.rp_exact = exact
     g, o, gs, os_ = run_pair(brp, cfg, b, steps=120)
     err = report(f"bubbly prec={prec}", g, o, FIELDS + ("fg",))
     rt = rel_l2(g.bubbles("R_t"), o.bubbles("R_t"))
@@ -95,7 +111,7 @@ def test_cavitating_bubbles(brp, prec):
     for n in ("ibn_x", "ibn_y", "ibm_x", "ibm_y"):
         if prec == 0:
             np.testing.assert_array_equal(g.bubbles(n), o.bubbles(n))
-    if prec == 0:
+    if prec == 0 and exact:
         for n in ("PG", "PL_p", "PL_n", "PL_m", "d1R", "R_p", "R_n", "dt", "re", "Q_B"):
             a, r = g.bubbles(n), o.bubbles(n)
             assert rel_l2(a, r) < 1e-8, n
