@@ -1,0 +1,157 @@
+"""The `bubbles` driver end to end on the GPU: its rawdata/ files (ASCII tuples
+and SPH records) hold the fields the oracle computes for the same
+configuration file."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(cases.ROOT, "cuda-rp_amd", "bin", "bubbles")
+
+CONF = """[grid]
+x = 48
+y = 48
+lx = 4.8e-3
+ly = 4.8e-3
+[pml]
+x0 = 0
+x1 = 1
+y0 = 0
+y1 = 1
+grid_number = 8
+sigma = 0.1
+order = 2
+[simulation]
+cfl = 0.5
+dt0 = 0.1
+n_step_max = 60
+t_step_max = 0
+save_interval = 30
+order = 2
+smooth_delta_band = 2
+[transducer]
+amp = 0.6
+freq = 1.0e6
+focal_dist = 3.4e-3
+box_size = 0.6e-3
+cylindrical = 0
+on_wave = 1000
+off_wave = 0
+focused_wave_p = 1
+[bubbles]
+enable = 1
+fg0 = 1.0e-6
+R0 = 3.0e-6
+P0 = 101325
+T0 = 293.15
+gamma = 1.4
+liquid_surface_tension = 0.0725
+liquid_viscosity = 1.0e-3
+liquid_density = 1000
+K0 = 26.2e-3
+[debug]
+fg = 1
+T = 1
+p = 1
+v = 1
+bubbles = 1
+[extensions]
+focus_x = 2.4e-3
+"""
+
+
+def conf_as_config():
+    c = cases.abi.default_config()
+    c.update(grid_x=48, grid_y=48, grid_lx=4.8e-3, grid_ly=4.8e-3, pml_x0=0, pml_x1=1, pml_y0=0,
+             pml_y1=1, pml_grid_number=8, pml_sigma=0.1, pml_order=2, sim_cfl=0.5, sim_dt0=0.1,
+             sim_n_step_max=60, sim_save_interval=30, sim_order=2, sim_smooth_delta_band=2,
+             tr_amp=0.6, tr_freq=1.0e6, tr_focal_dist=3.4e-3, tr_box_size=0.6e-3, tr_on_wave=1000,
+             tr_focused_p=1, bub_enable=1, bub_fg0=1e-6, bub_R0=3e-6, bub_P0=101325,
+             bub_T0=293.15, bub_gamma=1.4, bub_sigma=0.0725, bub_mu=1e-3, bub_rho=1000,
+             bub_K0=26.2e-3, ext_focus_x=2.4e-3, ext_focus_x_set=1)
+    return c
+
+
+def read_tuples(path, X, Y):
+    a = np.loadtxt(path)
+    assert a.shape == ((X + 1) * (Y + 1), 3)
+    return a[:, 2].reshape(Y + 1, X + 1), a[:, 0].reshape(Y + 1, X + 1), a[:, 1].reshape(Y + 1, X + 1)
+
+
+def read_sph(path):
+    b = open(path, "rb").read()
+    vals = struct.unpack_from("<4i", b, 0)
+    assert vals == (8, 1, 1, 8)
+    dims = struct.unpack_from("<3i", b, 20)
+    step, time = struct.unpack_from("<if", b, 88)
+    n = dims[0] * dims[1]
+    data = np.frombuffer(b, dtype="<f4", count=n, offset=104).reshape(dims[1], dims[0])
+    return dims, step, time, data
+
+
+def oracle_at(steps):
+    o = ol.OracleSim(conf_as_config())
+    o.prepare()
+    out = {}
+    done = 0
+    for s in steps:
+        o.step(s - done)
+        done = s
+        out[s] = {n: o.field(n) for n in ("p0", "fg", "vx", "vy")}
+        out[s]["R_t"] = o.bubbles("R_t")
+        out[s]["x"] = o.bubbles("x")
+        out[s]["time"] = o.info()["time"]
+    return out
+
+
+def test_driver_ascii_rawdata(tmp_path):
+    conf = tmp_path / "conf.txt"
+    conf.write_text(CONF)
+    out = tmp_path / "raw"
+    r = subprocess.run([BIN, "-a", "-d", str(out), str(conf)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = oracle_at([30, 60])
+    X = Y = 48
+    for s in (30, 60):
+        p0, xs, ys = read_tuples(out / f"p0_step_{s:05d}.txt", X, Y)
+        np.testing.assert_allclose(xs[0, :], np.arange(X + 1) * 1e-4, rtol=1e-5)
+        np.testing.assert_allclose(ys[:, 0], np.arange(Y + 1) * 1e-4, rtol=1e-5)
+        want = ref[s]["p0"][:Y + 1, :X + 1]
+        # 6 significant digits (default ostream formatting)
+        np.testing.assert_allclose(p0, want, rtol=5e-6, atol=1e-6 * np.abs(want).max())
+        fg, _, _ = read_tuples(out / f"fg_step_{s:05d}.txt", X, Y)
+        np.testing.assert_allclose(fg, ref[s]["fg"][:Y + 1, :X + 1], rtol=5e-6, atol=1e-12)
+        vx, _, _ = read_tuples(out / f"vx_step_{s:05d}.txt", X, Y)
+        # vx is stored from i = -1: column i sits at index i + 1
+        want_vx = ref[s]["vx"][:Y + 1, 1:X + 2]
+        np.testing.assert_allclose(vx, want_vx, rtol=5e-6, atol=1e-6 * np.abs(want_vx).max())
+        br = np.loadtxt(out / f"bubble_radii_{s:05d}.txt")
+        np.testing.assert_allclose(br[:, 2], ref[s]["R_t"], rtol=5e-6)
+        np.testing.assert_allclose(br[:, 0], ref[s]["x"], rtol=5e-6)
+    mm = np.loadtxt(out / "p0_minmax.txt")
+    assert mm[1] == pytest.approx(max(ref[s]["p0"][:Y + 1, :X + 1].max() for s in (30, 60)), rel=1e-5)
+    assert "Finished in" in (out / "runtime.txt").read_text()
+
+
+def test_driver_sph_rawdata(tmp_path):
+    conf = tmp_path / "conf.txt"
+    conf.write_text(CONF)
+    out = tmp_path / "raw"
+    r = subprocess.run([BIN, "-s", "-d", str(out), str(conf)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = oracle_at([60])
+    dims, step, time, data = read_sph(out / "p0_step_00060.sph")
+    assert dims == (49, 49, 1) and step == 60
+    assert time == pytest.approx(ref[60]["time"], rel=1e-6)
+    want = ref[60]["p0"][:49, :49].astype(np.float32)
+    np.testing.assert_allclose(data, want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
+    assert not list(out.glob("*.txt")) or all("minmax" in p.name or "focal" in p.name or
+                                              "runtime" in p.name for p in out.glob("*.txt"))

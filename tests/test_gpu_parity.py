@@ -181,7 +181,7 @@ def test_wide_band_and_edge_bubbles(brp):
     bub = (x, y, np.full(len(x), 3e-6), None, 1e-5)
     g, o, gs, os_ = run_pair(brp, cfg, bub, steps=80)
     err = report("band4/edges", g, o, FIELDS + ("fg",))
-    assert err["p0"] < 1e-12 and err["fg"] < 1e-12
+    assert err["p0"] < 1e-12 and err["fg"] < 1e-9
 
 
 def test_error_paths(brp):
