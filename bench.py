@@ -106,8 +106,25 @@ def cpu_baseline(abi, cfg, bub, sample_steps, threads):
     return wall, st
 
 
+def calibrate_copy(mib=1024, reps=5):
+    """A plain device copy of known size (reads and writes `mib` MiB per
+    launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale
+    those counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4,
+                      "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain copy kernel of 1 GiB (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=300)
@@ -117,9 +134,12 @@ def main():
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--bubbles", type=int, default=1 << 20)
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the coupled step runs on one MI355X; see DESIGN.md for how it would shard)",
+              "(the coupled step runs on one MI355X)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -57,6 +57,8 @@ def load_library():
         "brp_save_state": (C.c_int, [vp, i32]),
         "brp_restore_state": (C.c_int, [vp, i32]),
         "brp_get_phase_times": (C.c_int, [vp, dp]),
+        "brp_adjoint": (C.c_int, [vp, C.POINTER(abi.Objective), dp, dp, dp, i64,
+                                  C.POINTER(abi.AdjointStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -108,6 +110,7 @@ class _Backend:
         amp = np.ascontiguousarray(amp, dtype=np.float64)
         phase = np.ascontiguousarray(phase, dtype=np.float64)
         self._check(self._f("set_source")(self.h, len(amp), _dptr(amp), _dptr(phase)))
+        self._n_elem = len(amp)
 
     def prepare(self):
         self._check(self._f("prepare")(self.h))
@@ -173,6 +176,26 @@ class Simulation(_Backend):
 
     def restore_state(self, slot):
         self._check(self.lib.brp_restore_state(self.h, slot))
+
+    def adjoint(self, box, n_steps, window, checkpoint_every=0, n_elem=None):
+        """brp_adjoint from the current state: J = mean p0^2 over the cell box
+        (i0, i1, j0, j1) and the last `window` of `n_steps` steps, with dJ/d
+        element amplitude, dJ/d element phase and dJ/d initial void-fraction
+        map.  The handle ends at the state after n_steps."""
+        if n_elem is None:
+            n_elem = getattr(self, "_n_elem", 0)
+        ob = abi.Objective(box[0], box[1], box[2], box[3], n_steps, window, checkpoint_every, 0)
+        ga = np.zeros(max(n_elem, 1))
+        gp = np.zeros(max(n_elem, 1))
+        nx, ny = self.shape("fg")[:2]
+        gf = np.zeros((ny, nx))
+        st = abi.AdjointStats()
+        self._check(self.lib.brp_adjoint(self.h, C.byref(ob), _dptr(ga), _dptr(gp), _dptr(gf), nx,
+                                         C.byref(st)))
+        return {"J": st.J, "grad_amp": ga[:n_elem], "grad_phase": gp[:n_elem], "grad_fg0": gf,
+                "forward_ms": st.forward_ms, "reverse_ms": st.reverse_ms,
+                "memory_bytes": st.memory_bytes, "n_checkpoints": st.n_checkpoints,
+                "segment": st.segment}
 
     def phase_times(self):
         a = (C.c_double * 5)()

@@ -90,6 +90,18 @@ class Info(C.Structure):
     ]
 
 
+class Objective(C.Structure):
+    """brp_objective: mean p0^2 over a cell box and the last `window` steps."""
+    _fields_ = [(n, C.c_int32) for n in ("box_i0", "box_i1", "box_j0", "box_j1", "n_steps",
+                                          "window", "checkpoint_every", "pad_")]
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("J", C.c_double), ("forward_ms", C.c_double), ("reverse_ms", C.c_double),
+                ("memory_bytes", C.c_int64), ("n_checkpoints", C.c_int32),
+                ("segment", C.c_int32), ("pad_", C.c_int64)]
+
+
 def default_config():
     """Python twin of brp_config_default(): the reference's declared defaults
     (src/bubbles.cpp:97-132; calloc'd structs are zero) plus run controls."""

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -34,6 +35,32 @@ struct Snapshot {
     Window W;
     int64_t nstep;
     double t, tx;
+};
+
+// Intermediates of one replayed step, kept for its adjoint: the start-of-step
+// pressure and positions, every pressure evaluation's p0, every void-fraction
+// iterate (k = 0 is the prediction), and each Picard iteration's gathered
+// pressure and clamped radius.
+struct StepRecord {
+    void *p0_start = nullptr;
+    double *x_start = nullptr, *y_start = nullptr;
+    std::vector<void *> p0_it;       // [k] after evaluation k (k = 0..n_pic)
+    std::vector<int64_t *> fg_it;    // [k] void-fraction iterate k
+    std::vector<double *> PLp_it;    // [k] gathered PL_p of iteration k >= 1
+    std::vector<double *> Rt_it;     // [k] R_t of iteration k >= 1
+    std::vector<void *> owned;
+    // scalars of the recorded step
+    int n_pic = 0, first = 0, cur = 0, pl_m = 0, pl_n = 0, f_cur = 0;
+    double t = 0.0;
+};
+
+// Optional control of run_steps: a forced Picard count per step (the frozen
+// schedule of a replay), a record of one step's intermediates, and the
+// Picard counts taken.
+struct StepCtl {
+    const int *force_npic = nullptr;
+    StepRecord *rec = nullptr;
+    std::vector<int> *npic_out = nullptr;
 };
 
 }  // namespace
@@ -374,9 +401,148 @@ void field_extent(const brp_handle *h, int field, int *nx, int *ny, int *i0, int
     }
 }
 
+
+// ---- state snapshots (save/restore slots, adjoint checkpoints and tape) ----
+int snap_save(brp_handle *h, Snapshot &sn) {
+    if (!sn.valid && sn.bufs.empty()) {
+        for (size_t k = 0; k < h->allocs.size(); k++) {
+            void *p = nullptr;
+            if (hipMalloc(&p, h->alloc_sizes[k]) != hipSuccess) {
+                for (void *q : sn.bufs) (void)hipFree(q);
+                sn.bufs.clear();
+                sn.sizes.clear();
+                return set_err(h, BRP_ERR_DEVICE, "snapshot allocation failed (%zu bytes)",
+                               h->dev_bytes);
+            }
+            sn.bufs.push_back(p);
+            sn.sizes.push_back(h->alloc_sizes[k]);
+        }
+    }
+    if (sn.bufs.size() != h->allocs.size()) return set_err(h, BRP_ERR_STATE, "snapshot layout changed");
+    for (size_t k = 0; k < h->allocs.size(); k++)
+        HIPCHK(h, hipMemcpyAsync(sn.bufs[k], h->allocs[k], sn.sizes[k], hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    sn.pc = h->pc;
+    sn.fcur = h->fcur;
+    sn.fn = h->fn;
+    sn.fm = h->fm;
+    sn.fz = h->fz;
+    sn.rp_cur = h->rp_cur;
+    sn.pl_m = h->pl_m;
+    sn.pl_n = h->pl_n;
+    sn.pl_p = h->pl_p;
+    sn.first_step = h->first_step;
+    sn.W = h->W;
+    sn.nstep = h->nstep;
+    sn.t = h->t;
+    sn.tx = h->tx;
+    sn.valid = true;
+    return BRP_OK;
+}
+
+int snap_restore(brp_handle *h, const Snapshot &sn) {
+    if (sn.bufs.size() != h->allocs.size()) return set_err(h, BRP_ERR_STATE, "snapshot layout changed");
+    for (size_t k = 0; k < h->allocs.size(); k++)
+        HIPCHK(h, hipMemcpyAsync(h->allocs[k], sn.bufs[k], sn.sizes[k], hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->pc = sn.pc;
+    h->fcur = sn.fcur;
+    h->fn = sn.fn;
+    h->fm = sn.fm;
+    h->fz = sn.fz;
+    h->rp_cur = sn.rp_cur;
+    h->pl_m = sn.pl_m;
+    h->pl_n = sn.pl_n;
+    h->pl_p = sn.pl_p;
+    h->first_step = sn.first_step;
+    h->W = sn.W;
+    h->nstep = sn.nstep;
+    h->t = sn.t;
+    h->tx = sn.tx;
+    return BRP_OK;
+}
+
+void snap_free(Snapshot &sn) {
+    for (void *q : sn.bufs) (void)hipFree(q);
+    sn.bufs.clear();
+    sn.sizes.clear();
+    sn.valid = false;
+}
+
+size_t snap_bytes(const brp_handle *h) {
+    size_t b = 0;
+    for (size_t z : h->alloc_sizes) b += z;
+    return b;
+}
+
+// Device memory owned by one adjoint call (freed at its end).
+struct AdjMem {
+    std::vector<void *> p;
+    size_t bytes = 0;
+    ~AdjMem() {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <typename T>
+    int get(brp_handle *h, T **out, size_t n) {
+        void *q = nullptr;
+        const size_t z = std::max<size_t>(n * sizeof(T), 256);
+        if (hipMalloc(&q, z) != hipSuccess)
+            return set_err(h, BRP_ERR_DEVICE, "adjoint allocation of %zu bytes failed", z);
+        if (hipMemsetAsync(q, 0, z, h->stream) != hipSuccess)
+            return set_err(h, BRP_ERR_DEVICE, "hipMemsetAsync failed");
+        p.push_back(q);
+        bytes += z;
+        *out = (T *)q;
+        return BRP_OK;
+    }
+};
+
+int rec_grow(brp_handle *h, StepRecord *r, int k) {
+    // k evaluations recorded so far need p0_it/fg_it[0..k-1] and PLp_it/Rt_it[1..k-1]
+    const size_t fb = h->fsz * h->frame;
+    auto get = [&](size_t bytes, void **out) -> int {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(bytes, 256)) != hipSuccess)
+            return set_err(h, BRP_ERR_DEVICE, "step record allocation failed");
+        r->owned.push_back(q);
+        *out = q;
+        return BRP_OK;
+    };
+    int rc;
+    if (!r->p0_start) {
+        void *a, *b, *c;
+        if ((rc = get(fb, &a)) || (rc = get(sizeof(double) * std::max(h->nb, 1), &b)) ||
+            (rc = get(sizeof(double) * std::max(h->nb, 1), &c)))
+            return rc;
+        r->p0_start = a;
+        r->x_start = (double *)b;
+        r->y_start = (double *)c;
+    }
+    while ((int)r->p0_it.size() < k) {
+        void *a, *b, *c = nullptr, *d = nullptr;
+        if ((rc = get(fb, &a)) || (rc = get(sizeof(int64_t) * h->frame, &b))) return rc;
+        if (!r->p0_it.empty() &&
+            ((rc = get(sizeof(double) * std::max(h->nb, 1), &c)) ||
+             (rc = get(sizeof(double) * std::max(h->nb, 1), &d))))
+            return rc;
+        r->p0_it.push_back(a);
+        r->fg_it.push_back((int64_t *)b);
+        r->PLp_it.push_back((double *)c);
+        r->Rt_it.push_back((double *)d);
+    }
+    return BRP_OK;
+}
+
+void rec_free(StepRecord &r) {
+    for (void *q : r.owned) (void)hipFree(q);
+    r = StepRecord{};
+}
+
 // ---- the coupled step ----------------------------------------------------
+int rec_grow(brp_handle *h, StepRecord *r, int k);
+
 template <typename F>
-int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
+int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullptr) {
     const double thr = (double)1.0e-7f;  // bubble_CUDA.cu:243
     const int max_pic = std::min(h->cfg.max_picard > 0 ? h->cfg.max_picard : 1000, kMaxEval - 1);
     const bool bub = h->cfg.bub_enable && h->nb > 0;
@@ -405,6 +571,23 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
             h->pl_p = m;
         }
         launch_zero_scalars(s, h->sc_d, max_pic + 1);
+        StepRecord *rec = ctl ? ctl->rec : nullptr;
+        const int forced = (ctl && ctl->force_npic) ? ctl->force_npic[step] : -1;
+        if (rec) {
+            int rc = rec_grow(h, rec, 1);
+            if (rc) return rc;
+            HIPCHK(h, hipMemcpyAsync(rec->p0_start, h->p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+            if (h->nb > 0) {
+                HIPCHK(h, hipMemcpyAsync(rec->x_start, h->b.x, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+                HIPCHK(h, hipMemcpyAsync(rec->y_start, h->b.y, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+            }
+            rec->first = h->first_step;
+            rec->cur = h->rp_cur;
+            rec->pl_m = h->pl_m;
+            rec->pl_n = h->pl_n;
+            rec->f_cur = old_cur;
+            rec->t = h->t;
+        }
         HIPCHK(h, hipEventRecord(h->ev[0], s));
         launch_velocity<F>(s, h->g, vx, vy, p0, h->fgb[old_cur], h->W, h->first_step, cxa, cxb,
                            cya, cyb);
@@ -418,6 +601,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
                                  h->rxp_d, h->sc_d);
         launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
         HIPCHK(h, hipEventRecord(h->ev[3], s));
+        if (rec) {
+            HIPCHK(h, hipMemcpyAsync(rec->p0_it[0], p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+            HIPCHK(h, hipMemcpyAsync(rec->fg_it[0], h->fgb[old_m], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
+        }
         HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr + sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIPCHK(h, hipStreamSynchronize(s));
         double r0;
@@ -431,25 +618,37 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
         int status = BRP_OK;
         int loop = 0;
         int target = h->fz, prev = old_m;
-        double bub_ms = 0.0, pic_p_ms = 0.0;
+        double bub_ms = 0.0, pic_p_ms = 0.0, other_ms = 0.0;
         if (bub) {
-            while (resimax > thr) {
+            while (forced >= 0 ? loop < forced : resimax > thr) {
                 if (loop >= max_pic) {
                     status = BRP_ERR_PICARD_CAP;
                     break;
                 }
                 loop++;
-                HIPCHK(h, hipEventRecord(h->ev[4], s));
+                if (rec) {
+                    int rc = rec_grow(h, rec, loop + 1);
+                    if (rc) return rc;
+                }
+                HIPCHK(h, hipEventRecord(h->ev[7], s));
                 launch_order_partition(s, h->b, h->bp.heavy_threshold);
+                HIPCHK(h, hipEventRecord(h->ev[4], s));
                 launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
                                       p0, h->fgb[target], h->sc_d, loop);
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
+                if (rec) {
+                    HIPCHK(h, hipMemcpyAsync(rec->fg_it[loop], h->fgb[target], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
+                    HIPCHK(h, hipMemcpyAsync(rec->PLp_it[loop], h->b.PL[h->pl_p], sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+                    HIPCHK(h, hipMemcpyAsync(rec->Rt_it[loop], h->b.R_t, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+                }
                 HIPCHK(h, hipEventRecord(h->ev[5], s));
                 launch_pressure_picard<F>(s, h->g, vx, vy, (F *)h->pb[pn_i][0], (F *)h->pb[pn_i][1],
                                           (F *)h->pb[p_i][0], (F *)h->pb[p_i][1], p0,
                                           h->fgb[target], h->fgb[old_cur], h->fgb[prev], h->W, ax,
                                           bx, ay, by, h->xu_d, h->rxp_d, h->sc_d, loop);
                 launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
+                if (rec)
+                    HIPCHK(h, hipMemcpyAsync(rec->p0_it[loop], p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
                 HIPCHK(h, hipEventRecord(h->ev[6], s));
                 HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr, hipMemcpyDeviceToHost, s));
                 HIPCHK(h, hipMemcpyAsync(&h->sc_h->resid[loop], &h->sc_d->resid[loop], 8,
@@ -459,9 +658,11 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
                 std::memcpy(&r, &h->sc_h->resid[loop], 8);
                 resimax = r / h->P_inf;
                 std::swap(target, prev);
-                float a = 0.f, c = 0.f;
+                float a = 0.f, c = 0.f, o = 0.f;
+                (void)hipEventElapsedTime(&o, h->ev[7], h->ev[4]);
                 (void)hipEventElapsedTime(&a, h->ev[4], h->ev[5]);
                 (void)hipEventElapsedTime(&c, h->ev[5], h->ev[6]);
+                other_ms += o;
                 bub_ms += a;
                 pic_p_ms += c;
                 if (h->sc_h->flags & 4) {
@@ -490,6 +691,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
             h->phase_ms[1] += b2;
             h->phase_ms[2] += c + pic_p_ms;
             h->phase_ms[3] += bub_ms;
+            h->phase_ms[4] += other_ms;
         }
         const int flags = h->sc_h->flags;
         if (status == BRP_OK && (flags & 3)) status = BRP_ERR_NONFINITE;
@@ -498,6 +700,8 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats) {
         st.time = h->t;
         st.resimax = resimax;
         st.n_pic = loop;
+        if (rec) rec->n_pic = loop;
+        if (ctl && ctl->npic_out) ctl->npic_out->push_back(loop);
         st.rp_max_substeps = h->sc_h->sub_max;
         st.rp_mean_substeps = (loop > 0 && h->nb > 0)
                                   ? (double)h->sc_h->sub_sum / ((double)h->nb * loop)
@@ -528,6 +732,205 @@ int alloc_grid(brp_handle *h) {
     return init_coefficients<F>(h);
 }
 
+
+// ---- discrete adjoint ------------------------------------------------------
+struct AdjState {
+    AdjField f{};
+    double *LfC = nullptr, *LfN = nullptr, *LfM = nullptr, *Lf = nullptr;  // f_g adjoints
+    double *gA = nullptr, *gP = nullptr;       // per source column
+    double *LR[4] = {}, *LRs[4] = {};          // RP state adjoints: step end / start
+    double *LA = nullptr, *LB = nullptr;       // PL history (PL_p, PL_n of the step end)
+    double *Lx = nullptr, *Ly = nullptr;
+    double *lRt = nullptr, *lPLp = nullptr, *lPLn = nullptr, *lPLm = nullptr;
+};
+
+// Reverse of one replayed step.  On entry `a` holds the adjoint of the state
+// after the step and the handle holds that state with `rec` its
+// intermediates; on exit `a` holds the adjoint of the state before it.
+template <typename F>
+void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, const int box[4],
+                  AdjState &a) {
+    hipStream_t s = h->stream;
+    const GridParams &g = h->g;
+    const bool bub = h->cfg.bub_enable && h->nb > 0;
+    const int n = rec.n_pic;
+    const int64_t *gC = h->fgb[rec.f_cur];
+    const F *ax = (const F *)h->ax, *bx = (const F *)h->bx, *ay = (const F *)h->ay,
+            *by = (const F *)h->by;
+    if (seed) launch_adj_seed<F>(s, g, (const F *)h->p0, a.f.p0, box, cJ);
+    // start-of-step f_g adjoints: f_gn' = f_g and f_gm' = f_gn of the start
+    double *LgC = a.LfN, *LfNs = a.LfM, *LfMs = a.LfC;
+    double *const none[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int k = n; k >= 1; k--) {
+        launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
+        launch_adj_pressure<F>(s, g, false, a.f, rec.fg_it[k], gC, k == n ? a.LfC : nullptr, a.Lf,
+                               LgC, nullptr, nullptr, ax, bx, ay, by);
+        launch_adj_scatter(s, g, h->bp, h->b, rec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
+        launch_adj_rp(s, g, h->bp, h->b, rec.cur, rec.pl_m, rec.pl_n, rec.PLp_it[k], a.lRt,
+                      k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn, a.lPLm);
+        launch_adj_gather<F>(s, g, h->bp, h->b, (const F *)rec.p0_it[k - 1], a.lPLp,
+                             k == n ? a.LA : nullptr, a.f.p0, a.Lx, a.Ly);
+    }
+    launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
+    launch_adj_pressure<F>(s, g, true, a.f, rec.fg_it[0], gC, n == 0 ? a.LfC : nullptr, nullptr,
+                           LgC, LfNs, LfMs, ax, bx, ay, by);
+    if (bub)
+        launch_adj_motion<F>(s, g, h->bp, h->nb, rec.x_start, rec.y_start, (const F *)h->vx,
+                             (const F *)h->vy, a.f.vx, a.f.vy, a.Lx, a.Ly);
+    launch_adj_velocity_ghosts(s, g, a.f.vx, a.f.vy);
+    launch_adj_velocity<F>(s, g, a.f.vx, a.f.vy, (const F *)rec.p0_start, gC, rec.first,
+                           (const F *)h->cxa, (const F *)h->cxb, (const F *)h->cya,
+                           (const F *)h->cyb, a.f.p0, LgC);
+    if (bub) {
+        launch_adj_pl_history(s, h->nb, n > 0, a.LA, a.LB, a.lPLn, a.lPLm);
+        if (n > 0)
+            for (int q = 0; q < 4; q++) {
+                std::swap(a.LR[q], a.LRs[q]);
+                launch_fill_d(s, a.LRs[q], (size_t)h->nb, 0.0);
+            }
+    }
+    std::swap(a.f.px, a.f.pnx);
+    std::swap(a.f.py, a.f.pny);
+    launch_fill_d(s, a.f.pnx, h->frame, 0.0);
+    launch_fill_d(s, a.f.pny, h->frame, 0.0);
+    a.LfC = LgC;
+    a.LfN = LfNs;
+    a.LfM = LfMs;
+}
+
+template <typename F>
+int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, double *grad_phase,
+                 double *grad_fg0, int64_t ld, brp_adjoint_stats *stats) {
+    using clk = std::chrono::steady_clock;
+    const int n = ob.n_steps;
+    const int K = ob.checkpoint_every > 0 ? std::min(ob.checkpoint_every, n)
+                                          : (int)std::ceil(std::sqrt((double)n));
+    const int nseg = (n + K - 1) / K;
+    const int box[4] = {ob.box_i0, ob.box_i1, ob.box_j0, ob.box_j1};
+    const double cJ =
+        1.0 / ((double)ob.window * (box[1] - box[0] + 1) * (double)(box[3] - box[2] + 1));
+    hipStream_t s = h->stream;
+    AdjMem mem;
+    std::vector<Snapshot> ck(nseg), tape(K);
+    Snapshot fin;
+    StepRecord rec;
+    struct Cleanup {
+        std::vector<Snapshot> *a, *b;
+        Snapshot *c;
+        StepRecord *r;
+        ~Cleanup() {
+            for (auto &x : *a) snap_free(x);
+            for (auto &x : *b) snap_free(x);
+            snap_free(*c);
+            rec_free(*r);
+        }
+    } cleanup{&ck, &tape, &fin, &rec};
+    int rc;
+    double *jacc = nullptr;
+    if ((rc = mem.get(h, &jacc, (size_t)n))) return rc;
+
+    // forward sweep with checkpoints; Picard counts recorded
+    const auto t0 = clk::now();
+    std::vector<int> npic;
+    npic.reserve(n);
+    StepCtl fctl;
+    fctl.npic_out = &npic;
+    for (int sg = 0; sg < nseg; sg++) {
+        if ((rc = snap_save(h, ck[sg]))) return rc;
+        const int s_end = std::min(n, (sg + 1) * K);
+        for (int st = sg * K + 1; st <= s_end; st++) {
+            if ((rc = run_steps<F>(h, 1, nullptr, &fctl))) return rc;
+            if (st > n - ob.window) launch_box_sumsq<F>(s, h->g, (const F *)h->p0, box, jacc + st - 1);
+        }
+    }
+    if ((rc = snap_save(h, fin))) return rc;
+    std::vector<double> jh(n);
+    HIPCHK(h, hipMemcpy(jh.data(), jacc, sizeof(double) * n, hipMemcpyDeviceToHost));
+    double J = 0.0;
+    for (int st = n - ob.window + 1; st <= n; st++) J += jh[st - 1];
+    J *= cJ;
+    const auto t1 = clk::now();
+
+    // adjoint state
+    AdjState a;
+    const size_t fr = h->frame, nb = (size_t)std::max(h->nb, 1);
+    double **fields[] = {&a.f.p0, &a.f.px, &a.f.py, &a.f.pnx, &a.f.pny, &a.f.vx, &a.f.vy,
+                         &a.LfC, &a.LfN, &a.LfM, &a.Lf};
+    for (double **q : fields)
+        if ((rc = mem.get(h, q, fr))) return rc;
+    if ((rc = mem.get(h, &a.gA, (size_t)h->N + 2)) || (rc = mem.get(h, &a.gP, (size_t)h->N + 2)))
+        return rc;
+    double **bubs[] = {&a.LR[0], &a.LR[1], &a.LR[2], &a.LR[3], &a.LRs[0], &a.LRs[1], &a.LRs[2],
+                       &a.LRs[3], &a.LA, &a.LB, &a.Lx, &a.Ly, &a.lRt, &a.lPLp, &a.lPLn, &a.lPLm};
+    for (double **q : bubs)
+        if ((rc = mem.get(h, q, nb))) return rc;
+
+    // reverse sweep: per segment, a tape of step start states, then each step
+    // replayed with its intermediates recorded and reversed
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        if ((rc = snap_restore(h, ck[sg]))) return rc;
+        const int s0 = sg * K + 1, s1 = std::min(n, (sg + 1) * K);
+        for (int st = s0; st <= s1; st++) {
+            if ((rc = snap_save(h, tape[st - s0]))) return rc;
+            if (st < s1) {
+                StepCtl c;
+                c.force_npic = &npic[st - 1];
+                if ((rc = run_steps<F>(h, 1, nullptr, &c))) return rc;
+            }
+        }
+        for (int st = s1; st >= s0; st--) {
+            if (st < s1 && (rc = snap_restore(h, tape[st - s0]))) return rc;
+            StepCtl c;
+            c.force_npic = &npic[st - 1];
+            c.rec = &rec;
+            if ((rc = run_steps<F>(h, 1, nullptr, &c))) return rc;
+            if (rec.n_pic != npic[st - 1])
+                return set_err(h, BRP_ERR_STATE, "replay of step %d took %d Picard iterations, not %d",
+                               st, rec.n_pic, npic[st - 1]);
+            adjoint_step<F>(h, rec, st > n - ob.window, cJ, box, a);
+        }
+    }
+    HIPCHK(h, hipStreamSynchronize(s));
+    const auto t2 = clk::now();
+
+    // gradients
+    if (grad_fg0) {
+        std::vector<double> c(fr), nn(fr), m(fr);
+        HIPCHK(h, hipMemcpy(c.data(), a.LfC, sizeof(double) * fr, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(nn.data(), a.LfN, sizeof(double) * fr, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(m.data(), a.LfM, sizeof(double) * fr, hipMemcpyDeviceToHost));
+        for (int j = 0; j <= h->M + 1; j++)
+            for (int i = 0; i <= h->N + 1; i++) {
+                const int64_t o = frame_off(i, j, h->pitch);
+                grad_fg0[(int64_t)j * ld + i] = c[o] + nn[o] + m[o];
+            }
+    }
+    if ((grad_amp || grad_phase) && h->n_elem > 0) {
+        std::vector<double> gA(h->N + 2), gP(h->N + 2);
+        HIPCHK(h, hipMemcpy(gA.data(), a.gA, sizeof(double) * gA.size(), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(gP.data(), a.gP, sizeof(double) * gP.size(), hipMemcpyDeviceToHost));
+        std::vector<double> ea(h->n_elem, 0.0), ep(h->n_elem, 0.0);
+        const long n_ap = h->g.ap_hi - h->g.ap_lo + 1;
+        for (int i = std::max(1, h->g.ap_lo); i <= std::min(h->N, h->g.ap_hi); i++) {
+            const int e = (int)((long)(i - h->g.ap_lo) * h->n_elem / n_ap);
+            ea[e] += gA[i];
+            ep[e] += gP[i];
+        }
+        if (grad_amp) std::copy(ea.begin(), ea.end(), grad_amp);
+        if (grad_phase) std::copy(ep.begin(), ep.end(), grad_phase);
+    }
+    if (stats) {
+        stats->J = J;
+        stats->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        // step record: p0 and f_g per evaluation, PL_p and R_t per iteration
+        const size_t recb = rec.p0_it.size() * ((h->fsz + 8) * fr + 16 * nb) + (h->fsz * fr + 16 * nb);
+        stats->memory_bytes = (int64_t)((nseg + K + 1) * snap_bytes(h) + mem.bytes + recb);
+        stats->n_checkpoints = nseg;
+        stats->segment = K;
+    }
+    return snap_restore(h, fin);
+}
 }  // namespace
 
 extern "C" {
@@ -961,65 +1364,34 @@ int brp_save_state(brp_handle *h, int32_t slot) {
     if (!h || slot < 0 || slot >= 16) return BRP_ERR_INVALID;
     int rc = brp_prepare(h);
     if (rc) return rc;
-    Snapshot &sn = h->snaps[slot];
-    if (!sn.valid) {
-        for (size_t k = 0; k < h->allocs.size(); k++) {
-            void *p = nullptr;
-            if (hipMalloc(&p, h->alloc_sizes[k]) != hipSuccess) {
-                for (void *q : sn.bufs) (void)hipFree(q);
-                sn.bufs.clear();
-                sn.sizes.clear();
-                return set_err(h, BRP_ERR_DEVICE, "snapshot allocation failed");
-            }
-            sn.bufs.push_back(p);
-            sn.sizes.push_back(h->alloc_sizes[k]);
-        }
-    }
-    if (sn.bufs.size() != h->allocs.size()) return set_err(h, BRP_ERR_STATE, "snapshot layout changed");
-    for (size_t k = 0; k < h->allocs.size(); k++)
-        HIPCHK(h, hipMemcpyAsync(sn.bufs[k], h->allocs[k], sn.sizes[k], hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    sn.pc = h->pc;
-    sn.fcur = h->fcur;
-    sn.fn = h->fn;
-    sn.fm = h->fm;
-    sn.fz = h->fz;
-    sn.rp_cur = h->rp_cur;
-    sn.pl_m = h->pl_m;
-    sn.pl_n = h->pl_n;
-    sn.pl_p = h->pl_p;
-    sn.first_step = h->first_step;
-    sn.W = h->W;
-    sn.nstep = h->nstep;
-    sn.t = h->t;
-    sn.tx = h->tx;
-    sn.valid = true;
-    return BRP_OK;
+    return snap_save(h, h->snaps[slot]);
 }
 
 int brp_restore_state(brp_handle *h, int32_t slot) {
     if (!h || slot < 0 || slot >= 16) return BRP_ERR_INVALID;
     Snapshot &sn = h->snaps[slot];
     if (!sn.valid) return set_err(h, BRP_ERR_STATE, "slot %d holds no state", slot);
-    if (sn.bufs.size() != h->allocs.size()) return set_err(h, BRP_ERR_STATE, "snapshot layout changed");
-    for (size_t k = 0; k < h->allocs.size(); k++)
-        HIPCHK(h, hipMemcpyAsync(h->allocs[k], sn.bufs[k], sn.sizes[k], hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->pc = sn.pc;
-    h->fcur = sn.fcur;
-    h->fn = sn.fn;
-    h->fm = sn.fm;
-    h->fz = sn.fz;
-    h->rp_cur = sn.rp_cur;
-    h->pl_m = sn.pl_m;
-    h->pl_n = sn.pl_n;
-    h->pl_p = sn.pl_p;
-    h->first_step = sn.first_step;
-    h->W = sn.W;
-    h->nstep = sn.nstep;
-    h->t = sn.t;
-    h->tx = sn.tx;
-    return BRP_OK;
+    return snap_restore(h, sn);
+}
+
+int brp_adjoint(brp_handle *h, const brp_objective *obj, double *grad_amp, double *grad_phase,
+                double *grad_fg0, int64_t ld, brp_adjoint_stats *stats) {
+    if (!h || !obj) return BRP_ERR_INVALID;
+    const brp_objective &o = *obj;
+    if (o.n_steps < 1 || o.window < 1 || o.window > o.n_steps || o.checkpoint_every < 0)
+        return set_err(h, BRP_ERR_INVALID, "adjoint: need 1 <= window <= n_steps");
+    if (o.box_i0 < 1 || o.box_i1 > h->N || o.box_i0 > o.box_i1 || o.box_j0 < 1 ||
+        o.box_j1 > h->M || o.box_j0 > o.box_j1)
+        return set_err(h, BRP_ERR_INVALID, "adjoint: focal box outside the interior");
+    if (grad_fg0 && ld < h->N + 2) return set_err(h, BRP_ERR_INVALID, "adjoint: ld < N + 2");
+    if (h->cfg.tr_cylindrical || h->cfg.sim_order != 2 || h->cfg.thermal)
+        return set_err(h, BRP_ERR_INVALID,
+                       "adjoint: Cartesian, simulation.order = 2, thermal off only");
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    return h->prec == BRP_PREC_FP64
+               ? adjoint_impl<double>(h, o, grad_amp, grad_phase, grad_fg0, ld, stats)
+               : adjoint_impl<float>(h, o, grad_amp, grad_phase, grad_fg0, ld, stats);
 }
 
 int brp_get_phase_times(const brp_handle *h, double ms[5]) {

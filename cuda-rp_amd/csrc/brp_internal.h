@@ -133,4 +133,47 @@ void launch_copy_d(hipStream_t s, double *dst, const double *src, int n);
 void launch_bubble_index(hipStream_t s, const GridParams &g, const BubbleArrays &b,
                          double *out, int which);
 void launch_zero_scalars(hipStream_t s, StepScalars *sc, int nslots);
+
+// ---- adjoint (kernels_adjoint.hip) ----
+// Adjoint fields, FP64, same frame as the forward fields.
+struct AdjField {
+    double *p0, *px, *py, *pnx, *pny, *vx, *vy;
+};
+template <typename F>
+void launch_box_sumsq(hipStream_t s, const GridParams &g, const F *p0, const int box[4],
+                      double *out);
+template <typename F>
+void launch_adj_seed(hipStream_t s, const GridParams &g, const F *p0, double *Lp0,
+                     const int box[4], double c);
+void launch_adj_pressure_ghosts(hipStream_t s, const GridParams &g, double *L,
+                                const double *el_amp, const double *el_phase, double t,
+                                double *gA, double *gP);
+template <typename F>
+void launch_adj_pressure(hipStream_t s, const GridParams &g, bool first, const AdjField &a,
+                         const int64_t *fk, const int64_t *gC, const double *extra, double *Lf,
+                         double *LgC, double *LfN, double *LfM, const F *ax, const F *bx,
+                         const F *ay, const F *by);
+void launch_adj_velocity_ghosts(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy);
+template <typename F>
+void launch_adj_velocity(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy,
+                         const F *p0, const int64_t *fgn, int first, const F *cxa, const F *cxb,
+                         const F *cya, const F *cyb, double *Lp0, double *LfC);
+void launch_adj_scatter(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                        const BubbleArrays &b, const double *Rt, const double *Lf, double *lRt,
+                        double *Lx, double *Ly);
+template <typename F>
+void launch_adj_gather(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                       const BubbleArrays &b, const F *p0, const double *lPL,
+                       const double *lPL_extra, double *Lp0, double *Lx, double *Ly);
+template <typename F>
+void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp, int n,
+                       const double *xs, const double *ys, const F *vx, const F *vy, double *Lvx,
+                       double *Lvy, double *Lx, double *Ly);
+void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                   const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
+                   const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
+                   double *lPLn, double *lPLm);
+void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,
+                           double *lPLn, double *lPLm);
+void launch_fill_d(hipStream_t s, double *p, size_t n, double v);
 }  // namespace brp

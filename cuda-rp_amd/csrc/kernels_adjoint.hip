@@ -1,0 +1,683 @@
+// kernels_adjoint.hip -- discrete adjoint of the coupled step for gfx950.
+//
+// Reverse-mode kernels for the objective J = mean over a focal box and the
+// last K steps of p0^2, differentiated with respect to the per-element source
+// amplitudes and phases and the initial void-fraction map.  Each kernel is
+// the transpose of one forward kernel at the forward's recorded state:
+//   k_adj_pressure   MixturePressureKernel (+ VFPredictionKernel on eval 0)
+//   k_adj_pressure_ghosts / k_adj_velocity_ghosts   the boundary kernels,
+//                    their writes undone in reverse program order
+//   k_adj_velocity_a / _b   VelocityKernel
+//   k_adj_scatter / k_adj_gather / k_adj_motion   the bubble-grid coupling,
+//                    including the derivative of the smooth-delta weights with
+//                    respect to the bubble position (bins are piecewise
+//                    constant and not differentiated)
+//   k_adj_rp         Rayleigh-Plesset over one mixture step at the frozen
+//                    sub-step schedule, seven tangent lanes per bubble
+// Grid adjoints are FP64 in both precision modes; scatter-type transposes
+// (gather adjoint, motion adjoint) use FP64 atomics, every other kernel is a
+// gather and writes each cell from one thread.
+#include "brp_internal.h"
+#include "rp_tangent.h"
+
+#include <math.h>
+
+namespace brp {
+namespace {
+
+constexpr int kAX = 64, kAY = 4;
+
+__device__ inline double fgd(int64_t v) { return (double)v * kFgInvScale; }
+
+__device__ inline double wdelta(double c, double k, int pos, double h, double b) {
+    return c * (1.0 + cos(k * (((double)pos - 0.5) * h - b)));
+}
+// d/db of wdelta
+__device__ inline double wdelta_db(double c, double k, int pos, double h, double b) {
+    return c * k * sin(k * (((double)pos - 0.5) * h - b));
+}
+
+__device__ inline int bin_idx(double x, double rdx, double c) {
+#pragma clang fp contract(off)
+    double v = floor(x * rdx + c);
+    return (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
+}
+
+// s5 = -(f - g) / dt / (1 - (f + g) / 2) and its partial derivatives.
+__device__ inline void s5_partials(double f, double g, double dt, double *dfd, double *dgd) {
+    const double D = 1.0 - 0.5 * (f + g);
+    const double q = (f - g) / (dt * D * D);
+    *dfd = -1.0 / (dt * D) - 0.5 * q;
+    *dgd = 1.0 / (dt * D) - 0.5 * q;
+}
+
+// J accumulation: sum of p0^2 over the box into out[0] (one workgroup,
+// fixed reduction order).
+template <typename F>
+__global__ __launch_bounds__(256) void k_box_sumsq(GridParams g, const F *p0, int i0, int i1,
+                                                    int j0, int j1, double *out) {
+    const int ni = i1 - i0 + 1, n = ni * (j1 - j0 + 1);
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double v = (double)p0[frame_off(i0 + k % ni, j0 + k / ni, g.pitch)];
+        acc += v * v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// dJ/dp0 of the step's final pressure: 2 c p0 over the box.
+template <typename F>
+__global__ void k_adj_seed(GridParams g, const F *p0, double *Lp0, int i0, int i1, int j0,
+                           int j1, double c) {
+    const int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = j0 + (int)blockIdx.y;
+    if (i > i1 || j > j1) return;
+    const int64_t o = frame_off(i, j, g.pitch);
+    Lp0[o] += 2.0 * c * (double)p0[o];
+}
+
+// Adjoint of k_pressure_ghosts: part 2 undone first, then part 1 with each
+// column's writes in reverse order.  The focused source of element e,
+// A_e sin(omega s2 + phi_e), adds L * sin(.) to gA[i] and L * A_e cos(.) to
+// gP[i] of its column.
+__global__ __launch_bounds__(1024) void k_adj_pressure_ghosts(GridParams g, double *L,
+                                                               const double *el_amp,
+                                                               const double *el_phase, double t,
+                                                               double *gA, double *gP) {
+    const int N = g.N, M = g.M, P = g.pitch;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = tid; j <= M + 1; j += nt) {
+        if (!g.pml_x1) {
+            L[frame_off(N, j, P)] += L[frame_off(N + 1, j, P)];
+            L[frame_off(N + 1, j, P)] = 0.0;
+        }
+        if (!g.pml_x0) {
+            L[frame_off(1, j, P)] += L[frame_off(0, j, P)];
+            L[frame_off(0, j, P)] = 0.0;
+        }
+    }
+    __syncthreads();
+    for (int i = 1 + tid; i <= N; i += nt) {
+        if (!g.pml_y1) {
+            for (int j = N + 1; j >= N; j--) {
+                const int src = 2 * M + 1 - j;
+                if (j >= 0 && j <= M + 1 && src >= 0 && src <= M + 1) {
+                    L[frame_off(i, src, P)] += L[frame_off(i, j, P)];
+                    L[frame_off(i, j, P)] = 0.0;
+                }
+            }
+        }
+        const int64_t o0 = frame_off(i, 0, P);
+        if (g.src_row && g.focused_p) {
+            const double lam = L[o0];
+            const double xm = (i - 0.5) * g.dx, ym = -0.5 * g.dy;
+            double s1 = sqrt((g.fp_x - xm) * (g.fp_x - xm) + (g.fp_y - ym) * (g.fp_y - ym));
+            if (s1 <= g.fdist && el_amp && g.n_elem > 0 && g.ap_hi >= g.ap_lo) {
+                s1 = fmax(t - (g.fdist - s1) / g.cs_inf, 0.0);
+                const double s2 = fmod(s1, ((double)g.on_wave + (double)g.off_wave) / g.freq);
+                if (s2 < (double)g.on_wave / g.freq) {
+                    const long n_ap = g.ap_hi - g.ap_lo + 1;
+                    const int e = (int)((long)(i - g.ap_lo) * g.n_elem / n_ap);
+                    double sn, cn;
+                    sincos(g.omega * s2 + el_phase[e], &sn, &cn);
+                    gA[i] += lam * sn;
+                    gP[i] += lam * el_amp[e] * cn;
+                }
+            }
+            L[o0] = 0.0;
+        } else if (g.src_row && g.plane_p) {
+            L[o0] = 0.0;
+        }
+        if (!g.pml_y0) {
+            L[frame_off(i, 1, P)] += L[o0];
+            L[o0] = 0.0;
+        }
+    }
+}
+
+// Adjoint of one pressure evaluation p = P(pn, v, f, g) (k_pressure_first /
+// k_pressure_picard), as a gather over cells (i, j) in [-1, N+1] x [-1, M+1]:
+//   interior cells: gx = Lpx + Lp0, gy = Lpy + Lp0 ->  Lpn += g a,
+//     a = -s3 gx / bx, b = -s3 gy / by,  ds5 = (a + b) / 2
+//   vx node (i, j) gets  (a(i,j) - a(i+1,j)) rdx,  vy node (b(i,j) - b(i,j+1)) rdy
+//   f cells [0, N+1]^2:  Lf = ds5 ds5/df (+ extra),  LgC += ds5 ds5/dg.
+// FIRST: f is the prediction 3 fC - 3 fN + fM, so Lf goes to (LgC, LfN, LfM)
+// with weights (3, -3, 1) instead of to Lf (LfM is written, not added).
+template <typename F, bool FIRST>
+__global__ __launch_bounds__(256) void k_adj_pressure(
+    GridParams g, const double *__restrict__ Lp0, const double *__restrict__ Lpx,
+    const double *__restrict__ Lpy, double *__restrict__ Lpnx, double *__restrict__ Lpny,
+    double *__restrict__ Lvx, double *__restrict__ Lvy, const int64_t *__restrict__ fk,
+    const int64_t *__restrict__ gC, const double *extra, double *__restrict__ Lf,
+    double *__restrict__ LgC, double *__restrict__ LfN, double *LfM,
+    const F *__restrict__ ax, const F *__restrict__ bx, const F *__restrict__ ay,
+    const F *__restrict__ by) {
+    const int i = -1 + (int)(blockIdx.x * kAX + threadIdx.x);
+    const int j = -1 + (int)(blockIdx.y * kAY + threadIdx.y);
+    const int N = g.N, M = g.M, P = g.pitch;
+    if (i > N + 1 || j > M + 1) return;
+    const double s3 = (double)(F)g.s3;
+    const int64_t o = frame_off(i, j, P);
+    auto interior = [&](int a, int b) { return a >= 1 && a <= N && b >= 1 && b <= M; };
+    auto ax_of = [&](int a, int b) {
+        const int64_t q = frame_off(a, b, P);
+        return -s3 * (Lpx[q] + Lp0[q]) / (double)bx[a + 1];
+    };
+    auto by_of = [&](int a, int b) {
+        const int64_t q = frame_off(a, b, P);
+        return -s3 * (Lpy[q] + Lp0[q]) / (double)by[b + 1];
+    };
+    double a_c = 0.0, b_c = 0.0;
+    const bool in_c = interior(i, j);
+    if (in_c) {
+        const double gx = Lpx[o] + Lp0[o], gy = Lpy[o] + Lp0[o];
+        Lpnx[o] += gx * (double)ax[i + 1];
+        Lpny[o] += gy * (double)ay[j + 1];
+        a_c = -s3 * gx / (double)bx[i + 1];
+        b_c = -s3 * gy / (double)by[j + 1];
+    }
+    if (i >= 0 && i <= N && j >= 1 && j <= M) {
+        double v = a_c;
+        if (interior(i + 1, j)) v -= ax_of(i + 1, j);
+        Lvx[o] += v * (double)(F)g.rdx;
+    }
+    if (i >= 1 && i <= N && j >= 0 && j <= M) {
+        double v = b_c;
+        if (interior(i, j + 1)) v -= by_of(i, j + 1);
+        Lvy[o] += v * (double)(F)g.rdy;
+    }
+    if (i >= 0 && j >= 0) {
+        double lf = extra ? extra[o] : 0.0;
+        if (in_c) {
+            double df, dg;
+            s5_partials(fgd(fk[o]), fgd(gC[o]), g.dt, &df, &dg);
+            const double ds5 = 0.5 * (a_c + b_c);
+            lf += ds5 * df;
+            LgC[o] += ds5 * dg;
+        }
+        if (FIRST) {
+            LgC[o] += 3.0 * lf;
+            LfN[o] -= 3.0 * lf;
+            LfM[o] = lf;
+        } else {
+            Lf[o] = lf;
+        }
+    }
+}
+
+// The evaluation overwrote p, p0 in the interior: their adjoints are consumed.
+__global__ void k_adj_zero_interior(GridParams g, double *Lp0, double *Lpx, double *Lpy) {
+    const int i = 1 + (int)(blockIdx.x * kAX + threadIdx.x);
+    const int j = 1 + (int)(blockIdx.y * kAY + threadIdx.y);
+    if (i > g.N || j > g.M) return;
+    const int64_t o = frame_off(i, j, g.pitch);
+    Lp0[o] = 0.0;
+    Lpx[o] = 0.0;
+    Lpy[o] = 0.0;
+}
+
+// Adjoint of k_velocity_ghosts: parts VY b, VY a, VX b, VX a undone in turn.
+__global__ __launch_bounds__(1024) void k_adj_velocity_ghosts(GridParams g, double *Lvx,
+                                                               double *Lvy) {
+    const int N = g.N, M = g.M, P = g.pitch;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ double red[1024];
+    // VY part b: (0, j) <- (1, j); (N+1, j) <- q = vy(N, N) read before the part
+    double lq = 0.0;
+    for (int j = -1 + tid; j <= M + 1; j += nt) {
+        const int64_t oq = frame_off(N + 1, j, P);
+        lq += Lvy[oq];
+        Lvy[oq] = 0.0;
+        Lvy[frame_off(1, j, P)] += Lvy[frame_off(0, j, P)];
+        Lvy[frame_off(0, j, P)] = 0.0;
+    }
+    red[tid] = lq;
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0 && N <= M + 1) Lvy[frame_off(N, N, P)] += red[0];
+    __syncthreads();
+    // VY part a
+    for (int i = 1 + tid; i <= N; i += nt) {
+        if (!g.pml_y1) {
+            Lvy[frame_off(i, M - 1, P)] += Lvy[frame_off(i, M + 1, P)];
+            Lvy[frame_off(i, M + 1, P)] = 0.0;
+            Lvy[frame_off(i, M, P)] = 0.0;
+        }
+        if (g.src_row && (g.plane_v || g.focused_v)) Lvy[frame_off(i, 0, P)] = 0.0;
+        if (!g.pml_y0) {
+            Lvy[frame_off(i, 0, P)] += Lvy[frame_off(i, -1, P)];
+            Lvy[frame_off(i, -1, P)] = 0.0;
+            if (!g.plane_p && !g.focused_p) Lvy[frame_off(i, 0, P)] = 0.0;
+        }
+    }
+    __syncthreads();
+    // VX part b
+    for (int i = -1 + tid; i <= N + 1; i += nt) {
+        Lvx[frame_off(i, M, P)] += Lvx[frame_off(i, M + 1, P)];
+        Lvx[frame_off(i, M + 1, P)] = 0.0;
+        Lvx[frame_off(i, 1, P)] += Lvx[frame_off(i, 0, P)];
+        Lvx[frame_off(i, 0, P)] = 0.0;
+    }
+    __syncthreads();
+    // VX part a
+    for (int j = 1 + tid; j <= M; j += nt) {
+        if (!g.pml_x1) {
+            Lvx[frame_off(N, j, P)] = 0.0;
+            Lvx[frame_off(N - 1, j, P)] -= Lvx[frame_off(N + 1, j, P)];
+            Lvx[frame_off(N + 1, j, P)] = 0.0;
+        }
+        if (!g.pml_x0) {
+            Lvx[frame_off(0, j, P)] = 0.0;
+            Lvx[frame_off(1, j, P)] += Lvx[frame_off(-1, j, P)];
+            Lvx[frame_off(-1, j, P)] = 0.0;
+        }
+    }
+}
+
+// Adjoint of k_velocity, part a: p0 and f_gn adjoints as a gather over the
+// cells (i, j) in [0, N+1] x [0, M+1] from the vx nodes (i-1, j), (i, j) and
+// the vy nodes (i, j-1), (i, j).
+template <typename F>
+__global__ __launch_bounds__(256) void k_adj_velocity_a(
+    GridParams g, const double *__restrict__ Lvx, const double *__restrict__ Lvy,
+    const F *__restrict__ p0, const int64_t *__restrict__ fgn, int first,
+    const F *__restrict__ cxb, const F *__restrict__ cyb, double *__restrict__ Lp0,
+    double *__restrict__ LfC) {
+    const int i = (int)(blockIdx.x * kAX + threadIdx.x);
+    const int j = (int)(blockIdx.y * kAY + threadIdx.y);
+    const int N = g.N, M = g.M, P = g.pitch;
+    if (i > N + 1 || j > M + 1) return;
+    const double dt = (double)(F)g.dt, rdx = (double)(F)g.rdx, rho = g.rho_inf;
+    auto rho_at = [&](int a, int b) {
+        return first ? rho : (1.0 - fgd(fgn[frame_off(a, b, P)])) * rho;
+    };
+    auto pv = [&](int a, int b) { return (double)p0[frame_off(a, b, P)]; };
+    double lp = 0.0, lr = 0.0;  // adjoints of p0(i, j) and rho(i, j)
+    // vx node i' = i - 1: reads p0(i) with weight -, rho(i) in s1
+    if (i - 1 >= 0 && i - 1 <= N && j >= 1 && j <= M) {
+        const double c = Lvx[frame_off(i - 1, j, P)] / (double)cxb[i];
+        const double s1 = 0.5 * (rho_at(i - 1, j) + rho_at(i, j));
+        const double s2 = (pv(i, j) - pv(i - 1, j)) * rdx;
+        lp -= c * dt * rdx / s1;
+        lr += 0.5 * c * dt * s2 / (s1 * s1);
+    }
+    if (i <= N && j >= 1 && j <= M) {
+        const double c = Lvx[frame_off(i, j, P)] / (double)cxb[i + 1];
+        const double s1 = 0.5 * (rho_at(i, j) + rho_at(i + 1, j));
+        const double s2 = (pv(i + 1, j) - pv(i, j)) * rdx;
+        lp += c * dt * rdx / s1;
+        lr += 0.5 * c * dt * s2 / (s1 * s1);
+    }
+    if (i >= 1 && i <= N && j - 1 >= 0 && j - 1 <= M) {
+        const double c = Lvy[frame_off(i, j - 1, P)] / (double)cyb[j];
+        const double s1 = 0.5 * (rho_at(i, j - 1) + rho_at(i, j));
+        const double s2 = (pv(i, j) - pv(i, j - 1)) * rdx;  // rdx, as the forward
+        lp -= c * dt * rdx / s1;
+        lr += 0.5 * c * dt * s2 / (s1 * s1);
+    }
+    if (i >= 1 && i <= N && j <= M) {
+        const double c = Lvy[frame_off(i, j, P)] / (double)cyb[j + 1];
+        const double s1 = 0.5 * (rho_at(i, j) + rho_at(i, j + 1));
+        const double s2 = (pv(i, j + 1) - pv(i, j)) * rdx;
+        lp += c * dt * rdx / s1;
+        lr += 0.5 * c * dt * s2 / (s1 * s1);
+    }
+    const int64_t o = frame_off(i, j, P);
+    Lp0[o] += lp;
+    if (!first) LfC[o] -= rho * lr;
+}
+
+// Adjoint of k_velocity, part b: the updated nodes' own term vx cxa / cxb.
+template <typename F>
+__global__ void k_adj_velocity_b(GridParams g, double *Lvx, double *Lvy, const F *cxa,
+                                 const F *cxb, const F *cya, const F *cyb) {
+    const int i = (int)(blockIdx.x * kAX + threadIdx.x);
+    const int j = (int)(blockIdx.y * kAY + threadIdx.y);
+    if (i > g.N || j > g.M) return;
+    const int64_t o = frame_off(i, j, g.pitch);
+    if (j >= 1) Lvx[o] *= (double)cxa[i + 1] / (double)cxb[i + 1];
+    if (i >= 1) Lvy[o] *= (double)cya[j + 1] / (double)cyb[j + 1];
+}
+
+// Adjoint of the scatter of iteration k: lRt = dJ/dR_t, and the position
+// adjoints of the weights.
+__global__ __launch_bounds__(256) void k_adj_scatter(GridParams g, BubbleParams bp,
+                                                      BubbleArrays b, const double *Rt,
+                                                      const double *__restrict__ Lf, double *lRt,
+                                                      double *Lx, double *Ly) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= b.n) return;
+    const double x = b.x[idx], y = b.y[idx], R = Rt[idx];
+    const int ibnx = bin_idx(x, g.rdx, 0.5), ibny = bin_idx(y, g.rdy, 0.5);
+    const int nb = bp.mbe - bp.mbs + 1;
+    double s = 0.0, sx = 0.0, sy = 0.0;
+    for (int a = 0; a < nb; a++) {
+        const int i = ibnx + bp.mbs + a;
+        const double wx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+        const double dwx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+        for (int c = 0; c < nb; c++) {
+            const int j = ibny + bp.mbs + c;
+            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+                const double wy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
+                const double dwy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
+                const double l = Lf[frame_off(i, j, g.pitch)];
+                s += l * wx * wy;
+                sx += l * dwx * wy;
+                sy += l * wx * dwy;
+            }
+        }
+    }
+    const double base = bp.pi4r3 * b.n_B[idx] * g.rdx * g.rdy;  // fg_temp = base R^3
+    lRt[idx] = s * 3.0 * base * R * R;
+    const double ft = base * R * R * R;
+    Lx[idx] += ft * sx;
+    Ly[idx] += ft * sy;
+}
+
+// Adjoint of the gather PL_p = sum p0 w of iteration k (p0 = the previous
+// evaluation's pressure): FP64 atomics into Lp0, position adjoints.
+template <typename F>
+__global__ __launch_bounds__(256) void k_adj_gather(GridParams g, BubbleParams bp,
+                                                     BubbleArrays b, const F *__restrict__ p0,
+                                                     const double *lPL, const double *lPL_extra,
+                                                     double *Lp0, double *Lx, double *Ly) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= b.n) return;
+    const double lam = lPL[idx] + (lPL_extra ? lPL_extra[idx] : 0.0);
+    const double x = b.x[idx], y = b.y[idx];
+    const int ibnx = bin_idx(x, g.rdx, 0.5), ibny = bin_idx(y, g.rdy, 0.5);
+    const int nb = bp.mbe - bp.mbs + 1;
+    double sx = 0.0, sy = 0.0;
+    for (int a = 0; a < nb; a++) {
+        const int i = ibnx + bp.mbs + a;
+        const double wx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+        const double dwx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+        for (int c = 0; c < nb; c++) {
+            const int j = ibny + bp.mbs + c;
+            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+                const double wy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
+                const double dwy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
+                const int64_t o = frame_off(i, j, g.pitch);
+                if (lam != 0.0) atomicAdd(&Lp0[o], lam * wx * wy);
+                const double p = (double)p0[o];
+                sx += p * dwx * wy;
+                sy += p * wx * dwy;
+            }
+        }
+    }
+    Lx[idx] += lam * sx;
+    Ly[idx] += lam * sy;
+}
+
+// Adjoint of k_motion: x' = x + dt vLx(x, y; vx), y' = y + dt vLy(x, y; vy)
+// at the step's start positions (xs, ys); Lx, Ly hold dJ/dx', dJ/dy' and
+// become dJ/dx, dJ/dy.
+template <typename F>
+__global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams bp, int n,
+                                                     const double *xs, const double *ys,
+                                                     const F *__restrict__ vx,
+                                                     const F *__restrict__ vy, double *Lvx,
+                                                     double *Lvy, double *Lx, double *Ly) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const double x = xs[idx], y = ys[idx];
+    const double lx = Lx[idx], ly = Ly[idx], dt = g.dt;
+    const int ibmx = bin_idx(x, g.rdx, 1.0), ibmy = bin_idx(y, g.rdy, 1.0);
+    const int ibnx = bin_idx(x, g.rdx, 0.5), ibny = bin_idx(y, g.rdy, 0.5);
+    double dvx_dx = 0.0, dvx_dy = 0.0, dvy_dx = 0.0, dvy_dy = 0.0;
+    for (int i = ibmx + bp.nbs; i <= ibmx + bp.nbe; i++) {
+        const double Dx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+        const double dDx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+        for (int j = ibny + bp.mbs; j <= ibny + bp.mbe; j++) {
+            if (i >= -1 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+                const double Dy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
+                const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
+                const int64_t o = frame_off(i, j, g.pitch);
+                if (lx != 0.0) atomicAdd(&Lvx[o], dt * lx * Dx * Dy);
+                const double v = (double)vx[o];
+                dvx_dx += v * dDx * Dy;
+                dvx_dy += v * Dx * dDy;
+            }
+        }
+    }
+    for (int i = ibnx + bp.mbs; i <= ibnx + bp.mbe; i++) {
+        const double Dx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+        const double dDx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+        for (int j = ibmy + bp.nbs; j <= ibmy + bp.nbe; j++) {
+            if (i >= 0 && i <= g.N + 1 && j >= -1 && j <= g.M + 1) {
+                const double Dy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
+                const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
+                const int64_t o = frame_off(i, j, g.pitch);
+                if (ly != 0.0) atomicAdd(&Lvy[o], dt * ly * Dx * Dy);
+                const double v = (double)vy[o];
+                dvy_dx += v * dDx * Dy;
+                dvy_dy += v * Dx * dDy;
+            }
+        }
+    }
+    Lx[idx] = lx + dt * (lx * dvx_dx + ly * dvy_dx);
+    Ly[idx] = ly + dt * (lx * dvx_dy + ly * dvy_dy);
+}
+
+// Adjoint of one RP call (k_bubble_persistent's BubbleRadiusKernel part):
+// eight lanes per bubble, lane d < 7 carries the tangent of input d
+// (0..3 start state R, Rn, d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m) through the
+// frozen-schedule integration and ends with
+//   lam_d = sum_o LRe_o dOut_o/dIn_d + lRt dR_t/dIn_d.
+// LRe (the adjoint of the step's final RP state) is null except for the last
+// Picard iteration.  Outputs: LRs[d] += (d < 4), lPLp = (d = 4),
+// lPLn += (d = 5), lPLm += (d = 6).
+__global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, BubbleArrays b,
+                                                 int cur, int plm, int pln, const double *PLp,
+                                                 const double *lRt, const double *LRe0,
+                                                 const double *LRe1, const double *LRe2,
+                                                 const double *LRe3, double *LRs0, double *LRs1,
+                                                 double *LRs2, double *LRs3, double *lPLp,
+                                                 double *lPLn, double *lPLm) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = gid >> 3, d = gid & 7;
+    if (idx >= b.n) return;
+    const RPFastShared fs{bp.PL0,
+                          bp.gam,
+                          bp.mu,
+                          bp.rho,
+                          1.0 / bp.rho,
+                          bp.dt0,
+                          3.0 * bp.gam,
+                          3.0 * (bp.gam - 1.0),
+                          3.0 * (bp.gam - 1.0) * bp.K0 * bp.T0,
+                          2.0 * bp.sig,
+                          4.0 * bp.mu,
+                          bp.mu * bp.mu,
+                          -2.0 * bp.sig * bp.rho,
+                          4.0 * bp.pi * bp.K0,
+                          16.0 * bp.pi * bp.mu};
+    const double R0 = b.R0[idx], PG0 = b.PG0[idx];
+    const RPFastConst fk{R0,  1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R0 * R0 * R0,
+                         bp.T0 / (PG0 * R0 * R0 * R0)};
+    auto seed = [&](double v, int k) { return dd{v, d == k ? 1.0 : 0.0}; };
+    RPLaneT L;
+    rp_begin_t(L, seed(b.R[cur][idx], 0), seed(b.Rn[cur][idx], 1), seed(b.d1R[cur][idx], 2),
+               seed(b.PG[cur][idx], 3), b.bdt[cur][idx], b.re[cur][idx],
+               seed(b.PL[plm][idx], 6), seed(b.PL[pln][idx], 5), seed(PLp[idx], 4), g.dt,
+               bp.PL0);
+    while (L.remain > 0.0 && L.counter < bp.max_substeps) rp_substep_t(L, fk, fs);
+    double lam = lRt[idx] * L.Rt.d;
+    if (LRe0)
+        lam += LRe0[idx] * L.Rp.d + LRe1[idx] * L.Rn.d + LRe2[idx] * L.d1R.d +
+               LRe3[idx] * L.PG.d;
+    switch (d) {
+        case 0: LRs0[idx] += lam; break;
+        case 1: LRs1[idx] += lam; break;
+        case 2: LRs2[idx] += lam; break;
+        case 3: LRs3[idx] += lam; break;
+        case 4: lPLp[idx] = lam; break;
+        case 5: lPLn[idx] += lam; break;
+        case 6: lPLm[idx] += lam; break;
+        default: break;
+    }
+}
+
+// Liquid-pressure history adjoints across the step boundary:
+//   Picard ran:  LA <- lPLn + LB,  LB <- lPLm
+//   no Picard:   LA <- LA + LB,    LB <- 0        (PL_p was copied from PL_n)
+__global__ void k_adj_pl_history(int n, int picard, double *LA, double *LB, double *lPLn,
+                                 double *lPLm) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    if (picard) {
+        LA[idx] = lPLn[idx] + LB[idx];
+        LB[idx] = lPLm[idx];
+    } else {
+        LA[idx] = LA[idx] + LB[idx];
+        LB[idx] = 0.0;
+    }
+    lPLn[idx] = 0.0;
+    lPLm[idx] = 0.0;
+}
+
+__global__ void k_fill_d(double *p, size_t n, double v) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+}  // namespace
+
+// ---- launchers ----------------------------------------------------------
+
+template <typename F>
+void launch_box_sumsq(hipStream_t s, const GridParams &g, const F *p0, const int box[4],
+                      double *out) {
+    hipLaunchKernelGGL(k_box_sumsq<F>, dim3(1), dim3(256), 0, s, g, p0, box[0], box[1], box[2],
+                       box[3], out);
+}
+
+template <typename F>
+void launch_adj_seed(hipStream_t s, const GridParams &g, const F *p0, double *Lp0,
+                     const int box[4], double c) {
+    const int ni = box[1] - box[0] + 1, nj = box[3] - box[2] + 1;
+    hipLaunchKernelGGL(k_adj_seed<F>, dim3((ni + 63) / 64, nj), dim3(64), 0, s, g, p0, Lp0, box[0],
+                       box[1], box[2], box[3], c);
+}
+
+void launch_adj_pressure_ghosts(hipStream_t s, const GridParams &g, double *L,
+                                const double *el_amp, const double *el_phase, double t,
+                                double *gA, double *gP) {
+    hipLaunchKernelGGL(k_adj_pressure_ghosts, dim3(1), dim3(1024), 0, s, g, L, el_amp, el_phase,
+                       t, gA, gP);
+}
+
+template <typename F>
+void launch_adj_pressure(hipStream_t s, const GridParams &g, bool first, const AdjField &a,
+                         const int64_t *fk, const int64_t *gC, const double *extra, double *Lf,
+                         double *LgC, double *LfN, double *LfM, const F *ax, const F *bx,
+                         const F *ay, const F *by) {
+    const dim3 blk(kAX, kAY), grid((g.N + 3 + kAX - 1) / kAX, (g.M + 3 + kAY - 1) / kAY);
+    if (first)
+        hipLaunchKernelGGL((k_adj_pressure<F, true>), grid, blk, 0, s, g, a.p0, a.px, a.py, a.pnx,
+                           a.pny, a.vx, a.vy, fk, gC, extra, Lf, LgC, LfN, LfM, ax, bx, ay, by);
+    else
+        hipLaunchKernelGGL((k_adj_pressure<F, false>), grid, blk, 0, s, g, a.p0, a.px, a.py,
+                           a.pnx, a.pny, a.vx, a.vy, fk, gC, extra, Lf, LgC, LfN, LfM, ax, bx, ay,
+                           by);
+    const dim3 gi((g.N + kAX - 1) / kAX, (g.M + kAY - 1) / kAY);
+    hipLaunchKernelGGL(k_adj_zero_interior, gi, blk, 0, s, g, a.p0, a.px, a.py);
+}
+
+void launch_adj_velocity_ghosts(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy) {
+    hipLaunchKernelGGL(k_adj_velocity_ghosts, dim3(1), dim3(1024), 0, s, g, Lvx, Lvy);
+}
+
+template <typename F>
+void launch_adj_velocity(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy,
+                         const F *p0, const int64_t *fgn, int first, const F *cxa, const F *cxb,
+                         const F *cya, const F *cyb, double *Lp0, double *LfC) {
+    const dim3 blk(kAX, kAY);
+    const dim3 ga((g.N + 2 + kAX - 1) / kAX, (g.M + 2 + kAY - 1) / kAY);
+    hipLaunchKernelGGL(k_adj_velocity_a<F>, ga, blk, 0, s, g, Lvx, Lvy, p0, fgn, first, cxb, cyb,
+                       Lp0, LfC);
+    const dim3 gb((g.N + 1 + kAX - 1) / kAX, (g.M + 1 + kAY - 1) / kAY);
+    hipLaunchKernelGGL(k_adj_velocity_b<F>, gb, blk, 0, s, g, Lvx, Lvy, cxa, cxb, cya, cyb);
+}
+
+void launch_adj_scatter(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                        const BubbleArrays &b, const double *Rt, const double *Lf, double *lRt,
+                        double *Lx, double *Ly) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_adj_scatter, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, Rt, Lf,
+                       lRt, Lx, Ly);
+}
+
+template <typename F>
+void launch_adj_gather(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                       const BubbleArrays &b, const F *p0, const double *lPL,
+                       const double *lPL_extra, double *Lp0, double *Lx, double *Ly) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_adj_gather<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, p0,
+                       lPL, lPL_extra, Lp0, Lx, Ly);
+}
+
+template <typename F>
+void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp, int n,
+                       const double *xs, const double *ys, const F *vx, const F *vy, double *Lvx,
+                       double *Lvy, double *Lx, double *Ly) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_adj_motion<F>, dim3((n + 255) / 256), dim3(256), 0, s, g, bp, n, xs, ys,
+                       vx, vy, Lvx, Lvy, Lx, Ly);
+}
+
+void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                   const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
+                   const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
+                   double *lPLn, double *lPLm) {
+    if (b.n <= 0) return;
+    const long threads = (long)b.n * 8;
+    hipLaunchKernelGGL(k_adj_rp, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g, bp,
+                       b, cur, plm, pln, PLp, lRt, LRe[0], LRe[1], LRe[2], LRe[3], LRs[0], LRs[1],
+                       LRs[2], LRs[3], lPLp, lPLn, lPLm);
+}
+
+void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,
+                           double *lPLn, double *lPLm) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_adj_pl_history, dim3((n + 255) / 256), dim3(256), 0, s, n, picard, LA, LB,
+                       lPLn, lPLm);
+}
+
+void launch_fill_d(hipStream_t s, double *p, size_t n, double v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+#define BRP_ADJ_INSTANTIATE(F)                                                                  \
+    template void launch_box_sumsq<F>(hipStream_t, const GridParams &, const F *, const int[4], \
+                                      double *);                                                \
+    template void launch_adj_seed<F>(hipStream_t, const GridParams &, const F *, double *,      \
+                                     const int[4], double);                                     \
+    template void launch_adj_pressure<F>(hipStream_t, const GridParams &, bool, const AdjField &, \
+                                         const int64_t *, const int64_t *, const double *,      \
+                                         double *, double *, double *, double *, const F *,     \
+                                         const F *, const F *, const F *);                      \
+    template void launch_adj_velocity<F>(hipStream_t, const GridParams &, double *, double *,   \
+                                         const F *, const int64_t *, int, const F *, const F *, \
+                                         const F *, const F *, double *, double *);             \
+    template void launch_adj_gather<F>(hipStream_t, const GridParams &, const BubbleParams &,   \
+                                       const BubbleArrays &, const F *, const double *,         \
+                                       const double *, double *, double *, double *);           \
+    template void launch_adj_motion<F>(hipStream_t, const GridParams &, const BubbleParams &,   \
+                                       int, const double *, const double *, const F *,          \
+                                       const F *, double *, double *, double *, double *);
+BRP_ADJ_INSTANTIATE(float)
+BRP_ADJ_INSTANTIATE(double)
+
+}  // namespace brp

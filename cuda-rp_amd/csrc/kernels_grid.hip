@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_velocity(GridParams g, F *__restrict__ 
         }
         if (i >= 1) {
             const F s1 = (rc + rho_m_at<F>(g, fgn, w, first, i, j + 1)) * half;
-            const F s2 = (-pc + p0[o + g.pitch]) * rdx;  // rdx, as the reference (Q3)
+            const F s2 = (-pc + p0[o + g.pitch]) * rdx;  // rdx, as the reference (kernel.cuh:671)
             vy[o] = (vy[o] * cya[j + 1] - dt / s1 * s2) / cyb[j + 1];
         }
     }
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(1024) void k_velocity_ghosts(GridParams g, F *vx, F
     __syncthreads();
     // VY part b: rows j in [-1, M+1].  The right ghost column copies
     // vy(N, jsta2n + N + 1) = vy(N, N) in every row: the reference reads row
-    // vy_width*i with the loop variable i (bubble_CUDA_kernel.cuh:808, Q4).
+    // vy_width*i with the loop variable i (bubble_CUDA_kernel.cuh:808).
     const int qj = N;  // logical row jsta2n + (N + 1)
     const bool q_in = qj <= M + 1;
     F q = q_in ? vy[frame_off(N, qj, P)] : (F)0;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(1024) void k_pressure_ghosts(GridParams g, F *p0,
                     (F)focused_value(g, i, t, pamp, -0.5 * g.dy, el_amp, el_phase);
         }
         if (!g.pml_y1) {
-            // Q5: rows X..X+1 copy row 2Y+1-j (rows outside [0, M+1] skipped).
+            // As the reference (kernel.cuh:929-932): rows X..X+1 copy row 2Y+1-j (rows outside [0, M+1] skipped).
             for (int j = N; j <= N + 1; j++) {
                 int src = 2 * M + 1 - j;
                 if (j >= 0 && j <= M + 1 && src >= 0 && src <= M + 1)

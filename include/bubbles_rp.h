@@ -243,8 +243,43 @@ int brp_save_state(brp_handle *h, int32_t slot);
 int brp_restore_state(brp_handle *h, int32_t slot);
 
 /* Device time of the last brp_step call split by phase (ms): [0] velocity,
- * [1] bubble motion, [2] pressure, [3] gather+RP+scatter, [4] other. */
+ * [1] bubble motion, [2] pressure, [3] gather+RP+scatter, [4] other (ordering of the bubble work queue). */
 int brp_get_phase_times(const brp_handle *h, double ms[5]);
+
+/* Objective of brp_adjoint: J = mean of p0^2 over the cells
+ * [box_i0, box_i1] x [box_j0, box_j1] (inclusive, within 1..N x 1..M) and
+ * over the final p0 of each of the last `window` of `n_steps` steps.  The
+ * forward run keeps a checkpoint every `checkpoint_every` steps
+ * (0: ceil(sqrt(n_steps))) and recomputes each segment during the reverse
+ * sweep. */
+typedef struct brp_objective {
+    int32_t box_i0, box_i1, box_j0, box_j1;
+    int32_t n_steps, window, checkpoint_every, pad_;
+} brp_objective;
+
+typedef struct brp_adjoint_stats {
+    double J;
+    double forward_ms;     /* the checkpointed forward sweep, host wall time */
+    double reverse_ms;     /* recomputation + adjoint kernels, host wall time */
+    int64_t memory_bytes;  /* device bytes of checkpoints, tape and adjoint state */
+    int32_t n_checkpoints, segment;
+    int64_t pad_;
+} brp_adjoint_stats;
+
+/* Discrete adjoint of n_steps coupled steps from the current state (the
+ * state must be the start of the run whose parameters are differentiated).
+ * Returns J and its gradient with respect to the per-element source
+ * amplitudes and phases set by brp_set_source (grad_amp, grad_phase: n_elem
+ * doubles each, may be NULL) and to the initial void-fraction map
+ * (grad_fg0: (N+2) x (M+2) logical cells i, j in [0, N+1], row-major with
+ * leading dimension ld, may be NULL), where the map is f_g = f_gn = f_gm as
+ * set by brp_set_field(BRP_FIELD_FG).  The Picard counts and the RP sub-step
+ * schedule of the forward run are frozen (replayed, not differentiated).
+ * On return the handle holds the state after n_steps.  Cartesian, order 2,
+ * thermal off.  No reference counterpart (the reference has no adjoint). */
+int brp_adjoint(brp_handle *h, const brp_objective *obj, double *grad_amp,
+                double *grad_phase, double *grad_fg0, int64_t ld,
+                brp_adjoint_stats *stats);
 
 #ifdef __cplusplus
 }

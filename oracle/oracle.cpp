@@ -89,8 +89,11 @@ struct BubParams {  // per-bubble slice of bub_params_t (bubble_CUDA.h:129-147)
 };
 
 // bubble_CUDA_kernel.cuh:227-260
+// forced_dtp: when not null, the step size is taken from *forced_dtp instead
+// of the step control (frozen-schedule replay; not part of the reference).
 void solveRayleighPlesset(double *Rt, double *Rp, double *Rn, double *d1R, double PG,
-                          double PL, double *dt, double *remain, const BubParams &bp) {
+                          double PL, double *dt, double *remain, const BubParams &bp,
+                          const double *forced_dtp = nullptr) {
     double Rm, Rnd2R, dtm, dtp;
     Rm = *Rn;
     *Rn = *Rp;
@@ -108,6 +111,7 @@ void solveRayleighPlesset(double *Rt, double *Rp, double *Rn, double *d1R, doubl
         dtp = std::min(dtp, 0.1 * (*Rn) / std::sqrt(std::fabs(Rnd2R)));
         OPS(3);
     }
+    if (forced_dtp) dtp = *forced_dtp;
     *Rt = *Rp = (1.0 + dtp / dtm) * (*Rn) - dtp / dtm * Rm +
                 (dtp / (*Rn)) * (0.5 * (dtp + dtm) * Rnd2R);
     OPS(12);
@@ -208,8 +212,11 @@ struct RPIn {
 struct RPOut {
     double R_t, R_p, R_n, d1_R_p, PG_p, dt, re, Q_B;
 };
+// sched: when not null, the accepted step sizes are appended to it
+// (replay == false) or taken from it in order (replay == true).
 int rp_integrate(const BubParams &bp, double dt_mix, const RPIn &in, RPOut *out,
-                 int max_substeps, bool *capped) {
+                 int max_substeps, bool *capped, std::vector<double> *sched = nullptr,
+                 bool replay = false) {
     double PGn, PL, Rt = 0.0, omega_N;
     double dTdr_R = 0, SumHeat, SumVis;
     cplx alpha_N, Lp_N;
@@ -241,7 +248,13 @@ int rp_integrate(const BubParams &bp, double dt_mix, const RPIn &in, RPOut *out,
         PGn = PGp;
         PL = PC2 * time * time + PC1 * time + PC0;
         OPS(5);
-        solveRayleighPlesset(&Rt, &Rp, &Rn, &d1Rp, PGn, PL, &dt_L, &remain, bp);
+        if (replay && counter >= (int)sched->size()) {  // schedule exhausted
+            *capped = true;
+            break;
+        }
+        solveRayleighPlesset(&Rt, &Rp, &Rn, &d1Rp, PGn, PL, &dt_L, &remain, bp,
+                             replay ? &(*sched)[counter] : nullptr);
+        if (sched && !replay) sched->push_back(dt_L);
         time = time + dt_L;
         OPS(1);
         omega_N = solveOmegaN(&alpha_N, PGn, Rn, bp);
@@ -339,6 +352,13 @@ struct oracle {
         PL_n, PL_m, Q_B, n_B, bdt, bdt_n, re, re_n, vBx, vBy, vLx, vLy;
     std::vector<int> ibmx, ibmy, ibnx, ibny;
     std::vector<BubParams> bp;
+
+    // frozen-schedule record / replay (mode 1 / 2): Picard counts per step and
+    // the accepted RP step sizes of every (RP call, bubble)
+    int sched_mode = 0;
+    std::vector<int> sched_npic;
+    std::vector<std::vector<double>> sched_dt;
+    size_t sched_step = 0, sched_call = 0;
 
     // clock (bubble_CUDA.cu:68-69,208-212)
     int64_t nstep = 0;
@@ -655,7 +675,7 @@ void velocity_kernel(oracle *o) {
             }
             if (ok2) {
                 s1 = (o->rho_m(in, jn) + o->rho_m(in, jn + 1)) * 0.5;
-                s2 = (-o->p0(in, jn) + o->p0(in, jn + 1)) * o->rdx;  // rdx: reference Q3
+                s2 = (-o->p0(in, jn) + o->p0(in, jn + 1)) * o->rdx;  // rdx, as the reference (kernel.cuh:671)
                 s3 = (o->c_sl(in, jn) + o->c_sl(in, jn + 1)) * 0.5;
                 s3 = s3 * 0.5 * dtm * o->sig_ny[jn - a.jsta2n];
                 o->vy(in, jn) = (o->vy(in, jn) * (1.0 - s3) - dtm / s1 * s2) / (1.0 + s3);
@@ -711,7 +731,7 @@ void velocity_boundary(oracle *o) {
     // VY part b: rows j in [jsta2n, jend2n]
     for (int j = a.jsta2n; j <= a.jend2n; j++) {
         for (int i = 1 + a.ms + a.ns; i <= 0; i++) o->vy(i, j) = o->vy(1 - i, j);
-        // Reference Q4: the source row index is the loop variable i, so every
+        // The reference's source row index is the loop variable i, so every
         // row copies vy(X - (i - X - 1), jsta2n + i) = vy(X, Y) for order 2.
         for (int i = X + 1; i <= X + a.me + a.ne; i++) {
             int src_row_j = a.jsta2n + i;
@@ -721,7 +741,7 @@ void velocity_boundary(oracle *o) {
     }
 }
 
-// VelocityBoundaryKernel's Q4 read reaches row jsta2n + X + 1, which is inside
+// VelocityBoundaryKernel's right-ghost read (kernel.cuh:808) reaches row jsta2n + X + 1, which is inside
 // the vy array only while X + 1 <= jend2n - jsta2n, i.e. X <= Y + 1; beyond
 // that the reference reads out of bounds and this restatement reads 0.
 
@@ -816,7 +836,7 @@ void pressure_boundary(oracle *o) {
             if (c.tr_focused_p) o->p0(i, 0) = focused_source(o, i, t, pamp, -0.5 * o->dy, true);
         }
         if (!c.pml_y1) {
-            // Reference Q5: rows X..X+me copy row 2Y+1-j (rows outside the
+            // As the reference (kernel.cuh:929-932): rows X..X+me copy row 2Y+1-j (rows outside the
             // array are skipped).
             for (int j = X; j <= X + a.me; j++) {
                 int src = Y - (j - Y - 1 - a.jsta1m);
@@ -869,13 +889,26 @@ int solve_bubble_radii(oracle *o, double *mean_sub, bool *capped) {
     int64_t sum = 0;
     bool cap_any = false;
     const int cap = o->cfg.max_substeps > 0 ? o->cfg.max_substeps : 1000000;
+    std::vector<double> *sched = nullptr;
+    if (o->sched_mode) {
+        const size_t base = o->sched_call * (size_t)o->nb;
+        if (o->sched_mode == 1) o->sched_dt.resize(base + o->nb);
+        if (base + o->nb > o->sched_dt.size()) {
+            *mean_sub = 0.0;
+            *capped = true;
+            return 0;
+        }
+        sched = o->sched_dt.data() + base;
+        o->sched_call++;
+    }
 #pragma omp parallel for num_threads(o->nthreads) schedule(dynamic, 256) reduction(max : maxc) reduction(+ : sum) reduction(|| : cap_any)
     for (int b = 0; b < o->nb; b++) {
         RPIn in{o->R_pn[b], o->R_nn[b], o->d1_R_n[b], o->PG_n[b], o->bdt_n[b],
                 o->re_n[b], o->PL_m[b], o->PL_n[b], o->PL_p[b]};
         RPOut out;
         bool cp = false;
-        int cnt = rp_integrate(o->bp[b], o->dt, in, &out, cap, &cp);
+        int cnt = rp_integrate(o->bp[b], o->dt, in, &out, cap, &cp, sched ? sched + b : nullptr,
+                               o->sched_mode == 2);
         o->R_t[b] = out.R_t;
         o->R_p[b] = out.R_p;
         o->R_n[b] = out.R_n;
@@ -1164,7 +1197,15 @@ int oracle_step(oracle *o, int32_t k, brp_step_stats *stats) {
         double meansum = 0;
         if (o->cfg.bub_enable) {
             int loop = 0;
-            while (resimax > thr) {
+            int forced = -1;
+            if (o->sched_mode == 2) {
+                if (o->sched_step >= o->sched_npic.size()) {
+                    o->err = "frozen schedule exhausted";
+                    return BRP_ERR_STATE;
+                }
+                forced = o->sched_npic[o->sched_step++];
+            }
+            while (forced >= 0 ? loop < forced : resimax > thr) {
                 if (loop >= max_pic) {
                     status = BRP_ERR_PICARD_CAP;
                     break;
@@ -1181,6 +1222,7 @@ int oracle_step(oracle *o, int32_t k, brp_step_stats *stats) {
                 if (capped) break;
             }
             st.n_pic = loop;
+            if (o->sched_mode == 1) o->sched_npic.push_back(loop);
         }
         calculate_properties(o);
         if (status == BRP_OK && !finite_fields(o)) status = BRP_ERR_NONFINITE;
@@ -1198,6 +1240,27 @@ int oracle_step(oracle *o, int32_t k, brp_step_stats *stats) {
             return status;
         }
     }
+    return BRP_OK;
+}
+
+int oracle_schedule_mode(oracle *o, int32_t mode) {
+    if (!o || mode < 0 || mode > 2) return BRP_ERR_INVALID;
+    o->sched_mode = mode;
+    o->sched_step = 0;
+    o->sched_call = 0;
+    if (mode == 1) {
+        o->sched_npic.clear();
+        o->sched_dt.clear();
+    }
+    return BRP_OK;
+}
+
+int oracle_schedule_copy(oracle *dst, const oracle *src) {
+    if (!dst || !src || dst->nb != src->nb) return BRP_ERR_INVALID;
+    dst->sched_npic = src->sched_npic;
+    dst->sched_dt = src->sched_dt;
+    dst->sched_step = 0;
+    dst->sched_call = 0;
     return BRP_OK;
 }
 

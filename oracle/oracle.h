@@ -41,6 +41,14 @@ int oracle_get_bubbles(const oracle *o, int32_t which, double *dst);
 int oracle_gather_p0(oracle *o, double *PL_out);
 int oracle_scatter_values(oracle *o, const double *q, double *fg_out);
 const char *oracle_last_error(const oracle *o);
+/* Frozen schedule (for finite differences of a piecewise-smooth map):
+ * mode 1 records every step's Picard count and every accepted RP step size,
+ * mode 2 replays them in order instead of the residual test and the step
+ * control, mode 0 is the reference behaviour.  Setting a mode rewinds the
+ * replay position; mode 1 clears the record.  schedule_copy hands a record to
+ * another instance with the same bubble count. */
+int oracle_schedule_mode(oracle *o, int32_t mode);
+int oracle_schedule_copy(oracle *dst, const oracle *src);
 /* OpenMP threads used by oracle_step (1 = serial; result is identical). */
 void oracle_set_threads(oracle *o, int32_t n);
 

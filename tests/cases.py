@@ -80,3 +80,37 @@ def rel_l2(a, b):
 def rel_linf(a, b):
     m = np.max(np.abs(b))
     return np.max(np.abs(a - b)) / m if m > 0 else np.max(np.abs(a - b))
+
+
+def adjoint_case(n=48, nb=40, n_elem=8, seed=3, precision=abi.PREC_FP64, amp_pa=1.2e6):
+    """A bubbly case driven by an n_elem-element aperture (extension E4) with
+    seeded per-element amplitudes (0.6..1.4 x amp_pa) and phases (0..2 pi)."""
+    c, b = bubbly_case(n, nb, seed, precision, amp_pa)
+    r = rng(seed + 100)
+    amp = amp_pa * (0.6 + 0.8 * r.random(n_elem))
+    phase = 2.0 * np.pi * r.random(n_elem)
+    return c, b, (amp, phase)
+
+
+def focal_box(cfg, half=3):
+    """Cells (i0, i1, j0, j1), inclusive, of a (2 half) x (2 half) box centred
+    on the focus."""
+    dx = cfg.grid_lx / cfg.grid_x
+    fx = cfg.ext_focus_x
+    fy = cfg.tr_focal_dist * 0.5 * np.sqrt(3.0)
+    ic = int(round(fx / dx + 0.5))
+    jc = int(round(fy / dx + 0.5))
+    return ic - half, ic + half - 1, jc - half, jc + half - 1
+
+
+def focal_objective(sim, steps, box, window):
+    """Mean of p0^2 over the box cells and over the last `window` of `steps`
+    further steps (one step at a time; the p0 after each step counts)."""
+    i0, i1, j0, j1 = box
+    acc = 0.0
+    for s in range(steps):
+        sim.step(1)
+        if s >= steps - window:
+            p = sim.field("p0")[j0:j1 + 1, i0:i1 + 1]
+            acc += float(np.sum(p * p))
+    return acc / (window * (i1 - i0 + 1) * (j1 - j0 + 1))

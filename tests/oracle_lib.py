@@ -61,6 +61,8 @@ def load():
         "oracle_gather_p0": (C.c_int, [vp, dp]),
         "oracle_scatter_values": (C.c_int, [vp, dp, dp]),
         "oracle_rp_radius": (None, [vp, C.c_double, C.c_double, dp]),
+        "oracle_schedule_mode": (C.c_int, [vp, i32]),
+        "oracle_schedule_copy": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -103,6 +105,13 @@ class OracleSim(brp._Backend):
 
     def last_error(self):
         return self.lib.oracle_last_error(self.h).decode()
+
+    def schedule_mode(self, mode):
+        """0 reference behaviour, 1 record, 2 replay the recorded schedule."""
+        self._check(self.lib.oracle_schedule_mode(self.h, mode))
+
+    def schedule_from(self, other):
+        self._check(self.lib.oracle_schedule_copy(self.h, other.h))
 
     def array_index(self):
         out = (C.c_int32 * 28)()

@@ -25,13 +25,14 @@ def _declared_functions():
 def test_library_exports_every_declared_symbol(brp):
     lib = brp.load_library()
     names = _declared_functions()
-    assert len(names) >= 16
+    assert len(names) >= 17 and "brp_adjoint" in names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
 def _c_layout():
-    structs = {"brp_config": abi.Config, "brp_step_stats": abi.StepStats, "brp_info": abi.Info}
+    structs = {"brp_config": abi.Config, "brp_step_stats": abi.StepStats, "brp_info": abi.Info,
+               "brp_objective": abi.Objective, "brp_adjoint_stats": abi.AdjointStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

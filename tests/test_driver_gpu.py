@@ -88,10 +88,13 @@ def read_sph(path):
     b = open(path, "rb").read()
     vals = struct.unpack_from("<4i", b, 0)
     assert vals == (8, 1, 1, 8)
+    # records: [8][type][dtype][8] [12][dims][12] [12][origin][12] [12][pitch][12]
+    #          [8][step][time][8] [4n][data][4n]  (SphDataType.cpp:153-182)
     dims = struct.unpack_from("<3i", b, 20)
-    step, time = struct.unpack_from("<if", b, 88)
+    step, time = struct.unpack_from("<if", b, 80)
     n = dims[0] * dims[1]
-    data = np.frombuffer(b, dtype="<f4", count=n, offset=104).reshape(dims[1], dims[0])
+    assert struct.unpack_from("<i", b, 92)[0] == 4 * n and len(b) == 96 + 4 * n + 4
+    data = np.frombuffer(b, dtype="<f4", count=n, offset=96).reshape(dims[1], dims[0])
     return dims, step, time, data
 
 

@@ -81,8 +81,8 @@ def test_sample_boundaries_fp64(brp, cyl):
 
 def test_all_mirror_walls_and_plane_sources(brp):
     """No PML at all: every mirror branch of VelocityBoundaryKernel and
-    MixtureBoundaryPressureKernel runs, including the reference's quirks Q4
-    (right vy ghost column) and Q5 (y1 pressure mirror), with plane velocity
+    MixtureBoundaryPressureKernel runs, including the reference's right vy ghost
+    column (kernel.cuh:808) and y1 pressure mirror (kernel.cuh:929-932), with plane velocity
     and pressure sources on the y0 row."""
     cfg = cases.sample_like_case(40)
     cfg.update(grid_y=40, grid_ly=40e-4, pml_x1=0, pml_y1=0, tr_focused_p=0, tr_plane_p=1,

@@ -210,7 +210,7 @@ def test_pressure_source_closed_form():
     # rows 1..n-1: dp0 = rho c^2 dfg / (1 - mean fg)
     np.testing.assert_allclose(p0[1:n, 1:n + 1], want, rtol=1e-12)
     # row n: the reference's y1 mirror (pml.y1 = 0) copies ghost row n+1 (still 0)
-    # into interior row n (bubble_CUDA_kernel.cuh:929-932, quirk Q5)
+    # into interior row n (bubble_CUDA_kernel.cuh:929-932)
     np.testing.assert_array_equal(p0[n, 1:n + 1], 0.0)
     np.testing.assert_allclose(o.field("fg"), f + d, rtol=1e-15)
     np.testing.assert_allclose(o.field("fgn"), f, rtol=0)
@@ -358,7 +358,7 @@ def test_rp_free_oscillation_matches_model_ode():
     frequency, in bubble time.  Bubble time runs n_sub * dt0 per mixture step:
     when dt - 9 dt0 rounds above dt0, an 11th sub-step is clamped for R_t but
     the state and `time` advance by the unclamped dt0
-    (bubble_CUDA_kernel.cuh:248-256, 512) -- the reference's quirk Q21."""
+    (bubble_CUDA_kernel.cuh:248-256, 512)."""
     dt = 1e-4 * 0.5 / 1500.0
     R0, sig, mu, rho, PL0, gam, K0, T0 = 3e-6, 0.0725, 1e-3, 1000.0, 101325.0, 1.4, 26.2e-3, 293.15
     bp = ol.bubble_params(R0, dt_mix=dt)
@@ -406,7 +406,7 @@ def test_rp_flop_model_is_per_substep_linear():
 
 
 def test_picard_skipped_on_first_step():
-    """Q6: the first pressure evaluation of step 1 has zero residual, so RP
+    """The first pressure evaluation of step 1 has zero residual, so RP
     does not advance on that step (bubble_CUDA.cu:243)."""
     c, b = cases.bubbly_case(48, nb=40)
     o = cases.make(ol.OracleSim, c, b)
