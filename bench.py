@@ -57,6 +57,67 @@ def c2_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
     return cfg, (x, y, R0, None, fg_in)
 
 
+def c4_workload(abi, precision, n=1024, nb=65536, n_elem=64, seed=4):
+    """The adjoint workload: 1024^2, 65536 bubbles with R0 ~ U(1, 5) um in a
+    4 x 4 mm box at the focus (mean void fraction 1e-6 there), a 64-element
+    aperture with amplitudes ~ U(0.3, 0.7) MPa and phases ~ U(0, 2 pi)."""
+    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=0.5e6, freq=1.0e6, focal_dist=15e-3,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = np.random.default_rng(seed)
+    fpx, fpy = cfg.ext_focus_x, 15e-3 * 0.5 * math.sqrt(3.0)
+    box = 4e-3
+    x = fpx + (r.random(nb) - 0.5) * box
+    y = fpy + (r.random(nb) - 0.5) * box
+    R0 = 1e-6 + 4e-6 * r.random(nb)
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    amp = 0.3e6 + 0.4e6 * r.random(n_elem)
+    phase = 2.0 * math.pi * r.random(n_elem)
+    dx = cfg.grid_lx / n
+    ic, jc = int(round(fpx / dx + 0.5)), int(round(fpy / dx + 0.5))
+    focal_box = (ic - 10, ic + 9, jc - 10, jc + 9)  # 2 x 2 mm
+    return cfg, (x, y, R0, None, fg_in), (amp, phase), focal_box
+
+
+def adjoint_leg(brp, abi, steps, window, every):
+    """brp_adjoint on the adjoint workload next to a plain forward of the
+    same steps: adjoint/forward time."""
+    cfg, bub, src, box = c4_workload(abi, abi.PREC_MIXED)
+    f = brp.Simulation(cfg)
+    f.set_bubbles(*bub)
+    f.set_source(*src)
+    f.prepare()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        m = min(500, steps - done)
+        f.step(m)
+        done += m
+        log(f"  adjoint workload forward step {done}: {(time.perf_counter() - t0) / done * 1e3:.2f} ms/step")
+    t_fwd = time.perf_counter() - t0
+    f.close()
+    a = brp.Simulation(cfg)
+    a.set_bubbles(*bub)
+    a.set_source(*src)
+    a.prepare()
+    log(f"  adjoint: {steps} steps, window {window}, checkpoint every {every}")
+    t0 = time.perf_counter()
+    r = a.adjoint(box, steps, window, every)
+    t_adj = time.perf_counter() - t0
+    a.close()
+    g = np.concatenate([r["grad_amp"], r["grad_phase"]])
+    log(f"  adjoint: {t_adj:.2f} s vs forward {t_fwd:.2f} s")
+    return {"workload": "1024x1024 grid, 65536 bubbles R0~U(1,5) um in a 4x4 mm box, "
+                        "64 elements amp~U(0.3,0.7) MPa phase~U(0,2pi), mixed precision",
+            "steps": steps, "window": window, "checkpoint_every": every,
+            "J": r["J"], "forward_s": t_fwd, "adjoint_s": t_adj,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "adjoint_forward_sweep_s": r["forward_ms"] / 1e3,
+            "adjoint_reverse_sweep_s": r["reverse_ms"] / 1e3,
+            "memory_gb": r["memory_bytes"] / 1e9, "n_checkpoints": r["n_checkpoints"],
+            "grad_norm": float(np.linalg.norm(g)),
+            "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())}
+
+
 def stepped(sim, k, tag, chunk=25):
     """sim.step(k) in chunks, printing one progress line per chunk."""
     out = []
@@ -133,6 +194,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--bubbles", type=int, default=1 << 20)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=4000)
+    ap.add_argument("--adjoint-window", type=int, default=1000)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=200)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint leg (printed as one JSON line)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -146,6 +213,10 @@ def main():
     import importlib
     brp = importlib.import_module("cuda-rp_amd")
     abi = brp.abi
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(brp, abi, args.adjoint_steps, args.adjoint_window,
+                                     args.adjoint_checkpoint)), flush=True)
+        return
     n, nb = args.grid, args.bubbles
     first = min(args.cpu_steps, args.warmup)
 
@@ -215,6 +286,11 @@ def main():
         log(f"  fp64: {wall_d / args.steps * 1e3:.3f} ms/step; p0 err {err_p0:.2e}")
         sim_d.close()
     sim_m.close()
+
+    if not args.no_adjoint:
+        log("adjoint leg")
+        out["adjoint"] = adjoint_leg(brp, abi, args.adjoint_steps, args.adjoint_window,
+                                     args.adjoint_checkpoint)
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

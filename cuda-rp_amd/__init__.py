@@ -195,7 +195,7 @@ class Simulation(_Backend):
         return {"J": st.J, "grad_amp": ga[:n_elem], "grad_phase": gp[:n_elem], "grad_fg0": gf,
                 "forward_ms": st.forward_ms, "reverse_ms": st.reverse_ms,
                 "memory_bytes": st.memory_bytes, "n_checkpoints": st.n_checkpoints,
-                "segment": st.segment}
+                "segment": st.segment, "schedule_mismatch": st.schedule_mismatch}
 
     def phase_times(self):
         a = (C.c_double * 5)()

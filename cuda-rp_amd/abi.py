@@ -99,7 +99,7 @@ class Objective(C.Structure):
 class AdjointStats(C.Structure):
     _fields_ = [("J", C.c_double), ("forward_ms", C.c_double), ("reverse_ms", C.c_double),
                 ("memory_bytes", C.c_int64), ("n_checkpoints", C.c_int32),
-                ("segment", C.c_int32), ("pad_", C.c_int64)]
+                ("segment", C.c_int32), ("schedule_mismatch", C.c_int64)]
 
 
 def default_config():

@@ -48,6 +48,7 @@ struct StepRecord {
     std::vector<int64_t *> fg_it;    // [k] void-fraction iterate k
     std::vector<double *> PLp_it;    // [k] gathered PL_p of iteration k >= 1
     std::vector<double *> Rt_it;     // [k] R_t of iteration k >= 1
+    std::vector<int *> sub_it;       // [k] RP sub-steps of iteration k >= 1, per bubble
     std::vector<void *> owned;
     // scalars of the recorded step
     int n_pic = 0, first = 0, cur = 0, pl_m = 0, pl_n = 0, f_cur = 0;
@@ -292,6 +293,8 @@ int alloc_bubbles(brp_handle *h, int n) {
     if ((rc = dalloc(h, (void **)&h->b.sub, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.order, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.part, sizeof(int) * (size_t)((n + 255) / 256 + 2)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->b.bbox_blk, sizeof(int) * 4 * (size_t)((n + 255) / 256 + 1))))
+        return rc;
     return BRP_OK;
 }
 
@@ -519,16 +522,18 @@ int rec_grow(brp_handle *h, StepRecord *r, int k) {
         r->y_start = (double *)c;
     }
     while ((int)r->p0_it.size() < k) {
-        void *a, *b, *c = nullptr, *d = nullptr;
+        void *a, *b, *c = nullptr, *d = nullptr, *e = nullptr;
         if ((rc = get(fb, &a)) || (rc = get(sizeof(int64_t) * h->frame, &b))) return rc;
         if (!r->p0_it.empty() &&
             ((rc = get(sizeof(double) * std::max(h->nb, 1), &c)) ||
-             (rc = get(sizeof(double) * std::max(h->nb, 1), &d))))
+             (rc = get(sizeof(double) * std::max(h->nb, 1), &d)) ||
+             (rc = get(sizeof(int) * std::max(h->nb, 1), &e))))
             return rc;
         r->p0_it.push_back(a);
         r->fg_it.push_back((int64_t *)b);
         r->PLp_it.push_back((double *)c);
         r->Rt_it.push_back((double *)d);
+        r->sub_it.push_back((int *)e);
     }
     return BRP_OK;
 }
@@ -640,6 +645,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                     HIPCHK(h, hipMemcpyAsync(rec->fg_it[loop], h->fgb[target], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
                     HIPCHK(h, hipMemcpyAsync(rec->PLp_it[loop], h->b.PL[h->pl_p], sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
                     HIPCHK(h, hipMemcpyAsync(rec->Rt_it[loop], h->b.R_t, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+                    HIPCHK(h, hipMemcpyAsync(rec->sub_it[loop], h->b.sub, sizeof(int) * h->nb, hipMemcpyDeviceToDevice, s));
                 }
                 HIPCHK(h, hipEventRecord(h->ev[5], s));
                 launch_pressure_picard<F>(s, h->g, vx, vy, (F *)h->pb[pn_i][0], (F *)h->pb[pn_i][1],
@@ -742,6 +748,7 @@ struct AdjState {
     double *LA = nullptr, *LB = nullptr;       // PL history (PL_p, PL_n of the step end)
     double *Lx = nullptr, *Ly = nullptr;
     double *lRt = nullptr, *lPLp = nullptr, *lPLn = nullptr, *lPLm = nullptr;
+    unsigned int *mismatch = nullptr;  // RP replays whose sub-step count differed
 };
 
 // Reverse of one replayed step.  On entry `a` holds the adjoint of the state
@@ -767,7 +774,8 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, co
                                LgC, nullptr, nullptr, ax, bx, ay, by);
         launch_adj_scatter(s, g, h->bp, h->b, rec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
         launch_adj_rp(s, g, h->bp, h->b, rec.cur, rec.pl_m, rec.pl_n, rec.PLp_it[k], a.lRt,
-                      k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn, a.lPLm);
+                      k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn, a.lPLm, rec.sub_it[k],
+                      a.mismatch);
         launch_adj_gather<F>(s, g, h->bp, h->b, (const F *)rec.p0_it[k - 1], a.lPLp,
                              k == n ? a.LA : nullptr, a.f.p0, a.Lx, a.Ly);
     }
@@ -864,6 +872,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
                        &a.LRs[3], &a.LA, &a.LB, &a.Lx, &a.Ly, &a.lRt, &a.lPLp, &a.lPLn, &a.lPLm};
     for (double **q : bubs)
         if ((rc = mem.get(h, q, nb))) return rc;
+    if ((rc = mem.get(h, &a.mismatch, 1))) return rc;
 
     // reverse sweep: per segment, a tape of step start states, then each step
     // replayed with its intermediates recorded and reversed
@@ -919,7 +928,10 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         if (grad_amp) std::copy(ea.begin(), ea.end(), grad_amp);
         if (grad_phase) std::copy(ep.begin(), ep.end(), grad_phase);
     }
+    unsigned int mism = 0;
+    HIPCHK(h, hipMemcpy(&mism, a.mismatch, sizeof mism, hipMemcpyDeviceToHost));
     if (stats) {
+        stats->schedule_mismatch = mism;
         stats->J = J;
         stats->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         stats->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();

@@ -78,6 +78,7 @@ struct BubbleArrays {
     int *sub;             // sub-steps of the last RP call, per bubble
     int *order;           // work-queue order: bubbles with many sub-steps first
     int *part;            // partition scratch: per-block counts, then the heavy total
+    int *bbox_blk;        // per-block stencil bounding boxes of k_motion (4 ints each)
 };
 
 // Bubble physics constants shared by every bubble.
@@ -172,7 +173,7 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
 void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
                    const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
-                   double *lPLn, double *lPLm);
+                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch);
 void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,
                            double *lPLn, double *lPLm);
 void launch_fill_d(hipStream_t s, double *p, size_t n, double v);

@@ -18,7 +18,7 @@
 // (gather adjoint, motion adjoint) use FP64 atomics, every other kernel is a
 // gather and writes each cell from one thread.
 #include "brp_internal.h"
-#include "rp_tangent.h"
+#include "rp_fast.h"
 
 #include <math.h>
 
@@ -470,20 +470,24 @@ __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams b
 }
 
 // Adjoint of one RP call (k_bubble_persistent's BubbleRadiusKernel part):
-// eight lanes per bubble, lane d < 7 carries the tangent of input d
-// (0..3 start state R, Rn, d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m) through the
-// frozen-schedule integration and ends with
+// eight lanes per bubble, lane d < 7 runs the sub-step in (value, tangent)
+// arithmetic with the tangent seeded on input d (0..3 start state R, Rn,
+// d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m).  The value part repeats the forward's
+// operations exactly, so each lane takes the forward's sub-steps; lane 7
+// only counts them for the schedule check.  Each lane ends with
 //   lam_d = sum_o LRe_o dOut_o/dIn_d + lRt dR_t/dIn_d.
 // LRe (the adjoint of the step's final RP state) is null except for the last
 // Picard iteration.  Outputs: LRs[d] += (d < 4), lPLp = (d = 4),
-// lPLn += (d = 5), lPLm += (d = 6).
+// lPLn += (d = 5), lPLm += (d = 6); sub (if not null) holds the forward's
+// sub-step counts, and a lane-7 mismatch is counted in *mismatch.
 __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, BubbleArrays b,
                                                  int cur, int plm, int pln, const double *PLp,
                                                  const double *lRt, const double *LRe0,
                                                  const double *LRe1, const double *LRe2,
                                                  const double *LRe3, double *LRs0, double *LRs1,
                                                  double *LRs2, double *LRs3, double *lPLp,
-                                                 double *lPLn, double *lPLm) {
+                                                 double *lPLn, double *lPLm, const int *sub,
+                                                 unsigned int *mismatch) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int idx = gid >> 3, d = gid & 7;
     if (idx >= b.n) return;
@@ -502,16 +506,17 @@ __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, B
                           -2.0 * bp.sig * bp.rho,
                           4.0 * bp.pi * bp.K0,
                           16.0 * bp.pi * bp.mu};
-    const double R0 = b.R0[idx], PG0 = b.PG0[idx];
-    const RPFastConst fk{R0,  1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R0 * R0 * R0,
-                         bp.T0 / (PG0 * R0 * R0 * R0)};
-    auto seed = [&](double v, int k) { return dd{v, d == k ? 1.0 : 0.0}; };
-    RPLaneT L;
-    rp_begin_t(L, seed(b.R[cur][idx], 0), seed(b.Rn[cur][idx], 1), seed(b.d1R[cur][idx], 2),
-               seed(b.PG[cur][idx], 3), b.bdt[cur][idx], b.re[cur][idx],
-               seed(b.PL[plm][idx], 6), seed(b.PL[pln][idx], 5), seed(PLp[idx], 4), g.dt,
-               bp.PL0);
-    while (L.remain > 0.0 && L.counter < bp.max_substeps) rp_substep_t(L, fk, fs);
+    const double R0 = b.R0[idx];
+    const double PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
+    const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
+                         bp.T0 / (PG0 * R03)};
+    auto seed = [&](double v, int k) { return dd(v, d == k ? 1.0 : 0.0); };
+    RPLaneF<dd> L;
+    rp_begin_g<dd>(L, seed(b.R[cur][idx], 0), seed(b.Rn[cur][idx], 1), seed(b.d1R[cur][idx], 2),
+                   seed(b.PG[cur][idx], 3), b.bdt[cur][idx], b.re[cur][idx],
+                   seed(b.PL[plm][idx], 6), seed(b.PL[pln][idx], 5), seed(PLp[idx], 4), g.dt,
+                   bp.PL0);
+    while (L.remain > 0.0 && L.counter < bp.max_substeps) rp_substep_g(L, fk, fs);
     double lam = lRt[idx] * L.Rt.d;
     if (LRe0)
         lam += LRe0[idx] * L.Rp.d + LRe1[idx] * L.Rn.d + LRe2[idx] * L.d1R.d +
@@ -524,7 +529,9 @@ __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, B
         case 4: lPLp[idx] = lam; break;
         case 5: lPLn[idx] += lam; break;
         case 6: lPLm[idx] += lam; break;
-        default: break;
+        default:
+            if (sub && sub[idx] != L.counter) atomicAdd(mismatch, 1u);
+            break;
     }
 }
 
@@ -639,12 +646,12 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
 void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
                    const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
-                   double *lPLn, double *lPLm) {
+                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch) {
     if (b.n <= 0) return;
     const long threads = (long)b.n * 8;
     hipLaunchKernelGGL(k_adj_rp, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g, bp,
                        b, cur, plm, pln, PLp, lRt, LRe[0], LRe[1], LRe[2], LRe[3], LRs[0], LRs[1],
-                       LRs[2], LRs[3], lPLp, lPLn, lPLm);
+                       LRs[2], LRs[3], lPLp, lPLn, lPLm, sub, mismatch);
 }
 
 void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,

@@ -7,6 +7,7 @@
 // (the FP64 CAS loop of the reference, bubble_CUDA_kernel.cuh:95-114, is not).
 #include "brp_internal.h"
 #include "rp_device.h"
+#include "rp_fast.h"
 
 #include <math.h>
 
@@ -40,8 +41,11 @@ __device__ inline int64_t to_fixed(double v, bool *bad) {
     return (int64_t)__double2ll_rn(s);
 }
 
-// Block-wide reduction of (min, max) pairs into sc->bbox with four atomics.
-__device__ inline void block_bbox(int imin, int imax, int jmin, int jmax, StepScalars *sc) {
+// Block-wide reduction of (min, max) pairs into this block's slot of out
+// (4 ints); k_bbox_reduce folds the slots into sc->bbox.  One plain store per
+// block instead of device-scope atomics on four shared addresses, which
+// serialise across the whole grid.
+__device__ inline void block_bbox(int imin, int imax, int jmin, int jmax, int *out) {
     for (int off = 32; off > 0; off >>= 1) {
         imin = min(imin, __shfl_xor(imin, off, 64));
         imax = max(imax, __shfl_xor(imax, off, 64));
@@ -65,12 +69,41 @@ __device__ inline void block_bbox(int imin, int imax, int jmin, int jmax, StepSc
             jmin = min(jmin, red[2][k]);
             jmax = max(jmax, red[3][k]);
         }
-        if (imin <= imax) {
-            atomicMin(&sc->bbox[0], imin);
-            atomicMax(&sc->bbox[1], imax);
-            atomicMin(&sc->bbox[2], jmin);
-            atomicMax(&sc->bbox[3], jmax);
+        out[4 * blockIdx.x + 0] = imin;
+        out[4 * blockIdx.x + 1] = imax;
+        out[4 * blockIdx.x + 2] = jmin;
+        out[4 * blockIdx.x + 3] = jmax;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bbox_reduce(const int *blk, int nblk, StepScalars *sc) {
+    int imin = 0x7fffffff, imax = -0x7fffffff, jmin = 0x7fffffff, jmax = -0x7fffffff;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+        imin = min(imin, blk[4 * k + 0]);
+        imax = max(imax, blk[4 * k + 1]);
+        jmin = min(jmin, blk[4 * k + 2]);
+        jmax = max(jmax, blk[4 * k + 3]);
+    }
+    __shared__ int red[4][256];
+    red[0][threadIdx.x] = imin;
+    red[1][threadIdx.x] = imax;
+    red[2][threadIdx.x] = jmin;
+    red[3][threadIdx.x] = jmax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+            red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + s]);
         }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0][0] <= red[1][0]) {
+        sc->bbox[0] = red[0][0];
+        sc->bbox[1] = red[1][0];
+        sc->bbox[2] = red[2][0];
+        sc->bbox[3] = red[3][0];
     }
 }
 
@@ -87,21 +120,31 @@ __global__ __launch_bounds__(256) void k_motion(GridParams g, BubbleParams bp, B
         const double x = b.x[idx], y = b.y[idx];
         const int ibmx = bin_index(x, g.rdx, 1.0), ibmy = bin_index(y, g.rdy, 1.0);
         const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+        // the four weight sets (x at ibm / ibn columns, y at ibn / ibm rows),
+        // each evaluated once
+        const int nb = bp.mbe - bp.mbs + 1;
+        double wxm[kMaxBand], wyn[kMaxBand], wxn[kMaxBand], wym[kMaxBand];
+        for (int a = 0; a < nb; a++) {
+            wxm[a] = delta_w(bp.dcx, bp.dcy, ibmx + bp.nbs + a, g.dx, x);
+            wxn[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
+            wyn[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
+            wym[a] = delta_w(bp.dcx, bp.dcz, ibmy + bp.nbs + a, g.dy, y);
+        }
         double vLx = 0.0, vLy = 0.0;
-        for (int i = ibmx + bp.nbs; i <= ibmx + bp.nbe; i++) {
-            const double Dx = delta_w(bp.dcx, bp.dcy, i, g.dx, x);
-            for (int j = ibny + bp.mbs; j <= ibny + bp.mbe; j++) {
-                const double Dy = delta_w(bp.dcx, bp.dcz, j, g.dy, y);
+        for (int a = 0; a < nb; a++) {
+            const int i = ibmx + bp.nbs + a;
+            for (int c = 0; c < nb; c++) {
+                const int j = ibny + bp.mbs + c;
                 if (i >= -1 && i <= g.N + 1 && j >= 0 && j <= g.M + 1)
-                    vLx += (double)vx[frame_off(i, j, g.pitch)] * Dx * Dy;
+                    vLx += (double)vx[frame_off(i, j, g.pitch)] * wxm[a] * wyn[c];
             }
         }
-        for (int i = ibnx + bp.mbs; i <= ibnx + bp.mbe; i++) {
-            const double Dx = delta_w(bp.dcx, bp.dcy, i, g.dx, x);
-            for (int j = ibmy + bp.nbs; j <= ibmy + bp.nbe; j++) {
-                const double Dy = delta_w(bp.dcx, bp.dcz, j, g.dy, y);
+        for (int a = 0; a < nb; a++) {
+            const int i = ibnx + bp.mbs + a;
+            for (int c = 0; c < nb; c++) {
+                const int j = ibmy + bp.nbs + c;
                 if (i >= 0 && i <= g.N + 1 && j >= -1 && j <= g.M + 1)
-                    vLy += (double)vy[frame_off(i, j, g.pitch)] * Dx * Dy;
+                    vLy += (double)vy[frame_off(i, j, g.pitch)] * wxn[a] * wym[c];
             }
         }
         b.vx[idx] = vLx;
@@ -126,7 +169,7 @@ __global__ __launch_bounds__(256) void k_motion(GridParams g, BubbleParams bp, B
             jmax = -0x7fffffff;
         }
     }
-    block_bbox(imin, imax, jmin, jmax, sc);
+    block_bbox(imin, imax, jmin, jmax, b.bbox_blk);
 }
 
 // The scatter of one bubble's volume (VoidFractionReverseLookupKernel,
@@ -244,7 +287,8 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
     int idx = -1;                  // bubble of this lane, -1 when idle
     int batch_next = 0, batch_end = 0;  // wave-uniform
     bool exhausted = false;             // wave-uniform
-    RPLane L;
+    RPLane L;              // operation-by-operation path
+    RPLaneF<double> LF;    // shortened-chain path
     RPConst k{0.0, 0.0, 0.0, 0.0};
     double x = 0.0, y = 0.0;
     int ibnx = 0, ibny = 0;
@@ -291,25 +335,38 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
             if (FAST)
                 fk = RPFastConst{R0,        1.0 / R0,          k.PG0, 1.0 / k.PG0, k.calpha,
                                  k.PG0 * k.R03, bp.T0 / (k.PG0 * k.R03)};
-            RPState in{b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx],
-                       b.PG[cur][idx], b.bdt[cur][idx], b.re[cur][idx]};
-            rp_begin(L, in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, sh);
+            if (FAST) {
+                rp_begin_g(LF, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
+                           b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], PL,
+                           g.dt, bp.PL0);
+            } else {
+                RPState in{b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx],
+                           b.PG[cur][idx], b.bdt[cur][idx], b.re[cur][idx]};
+                rp_begin(L, in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, sh);
+            }
         }
         const bool active = idx >= 0;
         if (__ballot(active) == 0ull) break;
         // ---- one sub-step, or write back a bubble that reached the sync time
         if (active) {
-            const bool capped = L.counter >= bp.max_substeps;
-            if (L.remain > 0.0 && !capped) {
+            const int counter = FAST ? LF.counter : L.counter;
+            const bool capped = counter >= bp.max_substeps;
+            double remain = FAST ? LF.remain : L.remain;
+            if (remain > 0.0 && !capped) {
                 if (FAST)
-                    rp_substep_fast(L, fk, fs);
+                    rp_substep_g(LF, fk, fs);
                 else
                     rp_substep(L, k, sh);
+                remain = FAST ? LF.remain : L.remain;
             }
-            if (!(L.remain > 0.0) || capped) {
+            if (!(remain > 0.0) || capped) {
                 RPState out;
                 double R_t, Q_B;
-                rp_end(L, g.dt, &out, &R_t, &Q_B);
+                if (FAST)
+                    rp_end_g(LF, g.dt, &out, &R_t, &Q_B);
+                else
+                    rp_end(L, g.dt, &out, &R_t, &Q_B);
+                const int cnt = FAST ? LF.counter : L.counter;
                 b.R[nxt][idx] = out.Rp;
                 b.Rn[nxt][idx] = out.Rn;
                 b.d1R[nxt][idx] = out.d1R;
@@ -318,10 +375,10 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
                 b.re[nxt][idx] = out.re;
                 b.R_t[idx] = R_t;
                 b.Q_B[idx] = Q_B;
-                b.sub[idx] = L.counter;
-                sub_max = max(sub_max, L.counter);
-                sub_sum += (unsigned long long)L.counter;
-                capped_any |= capped && L.remain > 0.0;
+                b.sub[idx] = cnt;
+                sub_max = max(sub_max, cnt);
+                sub_sum += (unsigned long long)cnt;
+                capped_any |= capped && remain > 0.0;
                 bad |= !isfinite(R_t);
                 scatter_bubble(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], R_t, x),
                                fg_target, &bad);
@@ -395,8 +452,9 @@ template <typename F>
 void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, const F *vx, const F *vy, StepScalars *sc) {
     if (b.n <= 0) return;
-    hipLaunchKernelGGL(k_motion<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, vx, vy,
-                       sc);
+    const int nblk = (b.n + 255) / 256;
+    hipLaunchKernelGGL(k_motion<F>, dim3(nblk), dim3(256), 0, s, g, bp, b, vx, vy, sc);
+    hipLaunchKernelGGL(k_bbox_reduce, dim3(1), dim3(256), 0, s, b.bbox_blk, nblk, sc);
 }
 
 int bubble_step_blocks(int n_bubbles) {

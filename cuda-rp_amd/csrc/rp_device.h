@@ -211,16 +211,7 @@ __device__ inline void rp_end(const RPLane &L, double dt_mix, RPState *out, doub
     *Q_B = (L.SumHeat + L.SumVis) / (dt_mix - L.remain + L.re_n);
 }
 
-// ---- shortened dependency chain -------------------------------------------
-// The same model with fewer dependent divisions, square roots and
-// transcendentals per sub-step.  alpha = s (1 + i) always (rp_alpha), so
-//   coth(alpha) = (1 - |E|^2 + 2 i Im E) / |1 - E|^2,   E = exp(-2 alpha),
-// evaluated with expm1(-2s) and sincos(s) (no cancellation as s -> 0),
-//   1 / alpha^2 = -i / (2 s^2),
-//   Upsilon = 3 gam / (1 + 3 (gam - 1) u / alpha^2),  u = alpha coth(alpha) - 1,
-//   Lp = R (1/u + 1.5 i / s^2)   [= R (alpha^2 - 3u) / (alpha^2 u)],
-// and the reciprocals of R, rho, R0 and PG0 are shared.  Results equal the
-// reference operation order up to rounding.
+// ---- constants of the shortened-chain sub-step (rp_fast.h) ---------------
 struct RPFastConst {   // per bubble
     double R0, iR0, PG0, iPG0, calpha, PG0R03, T0_PG0R03;
 };
@@ -228,133 +219,5 @@ struct RPFastShared {  // shared by all bubbles
     double PL0, gam, mu, rho, irho, dt0, g3, k3, C0, two_sig, four_mu, mu2, m2sig_rho,
         four_pi_K0, sixteen_pi_mu;
 };
-
-__device__ inline cd alpha_coth_m1(double s) {
-#pragma clang fp contract(fast)
-    const double m = expm1(-2.0 * s);  // exp(-2s) - 1
-    double sn, cn;
-    sincos(s, &sn, &cn);
-    const double e2 = 1.0 + m;
-    const double s2 = 2.0 * sn * cn;         // sin 2s
-    const double c2 = 1.0 - 2.0 * sn * sn;   // cos 2s
-    const double ei = -e2 * s2;              // Im E
-    const double one_m_er = 2.0 * sn * sn - m * c2;  // 1 - Re E
-    const double iD = 1.0 / (one_m_er * one_m_er + ei * ei);
-    const double cr = -m * (m + 2.0) * iD;   // (1 - |E|^2) / |1 - E|^2
-    const double ci = 2.0 * ei * iD;
-    return cd{s * (cr - ci) - 1.0, s * (cr + ci)};
-}
-
-__device__ inline cd fast_upsilon(double sa, double gam, double k3) {
-#pragma clang fp contract(fast)
-    double tr, ti;
-    if (sa * 1.4142135623730951 > 1.0e-1) {
-        const cd u = alpha_coth_m1(sa);
-        const double h = 0.5 / (sa * sa);
-        tr = u.im * h;
-        ti = -u.re * h;
-    } else {  // series of (a coth a - 1) / a^2 with a^2 = 2 i s^2
-        const cd aa = c_mk(0.0, 2.0 * sa * sa);
-        cd u = c_rsub(2.0 / 945.0, c_divr(aa, 4725.0));
-        u = c_radd(-1.0 / 45.0, c_mul(aa, u));
-        u = c_radd(1.0 / 3.0, c_mul(aa, u));
-        tr = u.re;
-        ti = u.im;
-    }
-    const double dr = 1.0 + k3 * tr, di = k3 * ti;
-    const double f = 3.0 * gam / (dr * dr + di * di);
-    return cd{dr * f, -di * f};
-}
-
-__device__ inline cd fast_lp(double sa, double R) {
-#pragma clang fp contract(fast)
-    if (sa * 1.4142135623730951 > 1.0e-1) {
-        const cd u = alpha_coth_m1(sa);
-        const double iu = 1.0 / (u.re * u.re + u.im * u.im);
-        return cd{R * u.re * iu, R * (1.5 / (sa * sa) - u.im * iu)};
-    }
-    const cd aa = c_mk(0.0, 2.0 * sa * sa);
-    cd u = c_rsub(2.0 / 7875.0, c_divr(c_mulr(aa, 37.0), 3031875.0));
-    u = c_radd(-1.0 / 175.0, c_mul(aa, u));
-    u = c_radd(1.0 / 5.0, c_mul(aa, u));
-    return c_rmul(R, u);
-}
-
-__device__ inline void rp_substep_fast(RPLane &L, const RPFastConst &k, const RPFastShared &s) {
-#pragma clang fp contract(fast)
-    // solveRayleighPlesset
-    const double PGn = L.PG;
-    const double PL = L.PC2 * L.time * L.time + L.PC1 * L.time + L.PC0;
-    const double Rm = L.Rn;
-    const double Rn = L.Rp;
-    const double iRn = 1.0 / Rn;
-    double d1R = L.d1R;
-    const double Rnd2R =
-        (PGn - PL - s.two_sig * iRn - s.four_mu * iRn * d1R) * s.irho - 1.5 * d1R * d1R;
-    const double dtm = L.dtL;
-    double dtp = fmin(1.01 * dtm, s.dt0);
-    if (fabs(d1R) > kDblEps) dtp = fmin(dtp, 0.01 * Rn / fabs(d1R));
-    if (fabs(Rnd2R) > kDblEps) dtp = fmin(dtp, 0.1 * Rn / sqrt(fabs(Rnd2R)));
-    const double r = dtp / dtm;
-    const double acc = dtp * iRn * Rnd2R;
-    const double Rp = (1.0 + r) * Rn - r * Rm + acc * (0.5 * (dtp + dtm));
-    d1R = (dtm * dtm * Rp + (dtp * dtp - dtm * dtm) * Rn - dtp * dtp * Rm) /
-              (dtm * dtp * (dtm + dtp)) + acc;
-    const double dtL = dtp;
-    if (dtp >= L.remain) {
-        const double d = L.remain;
-        const double r2 = d / dtm;
-        L.Rt = (1.0 + r2) * Rn - r2 * Rm + d * iRn * Rnd2R * (0.5 * (d + dtm));
-        dtp = d;
-    } else {
-        L.Rt = Rp;
-    }
-    L.remain -= dtp;
-    L.time = L.time + dtL;
-    L.dtL = dtL;
-    L.Rp = Rp;
-    L.Rn = Rn;
-    L.d1R = d1R;
-    // solveOmegaN at (PGn, Rn)
-    const double Rx = Rn * k.iR0;
-    const double coef1 = k.PG0 / (PGn * Rx * Rx * Rx);
-    const double coef2 = PGn * s.rho;
-    const double coef3 = 4.0 * iRn * iRn;
-    const double value1 = s.m2sig_rho * iRn;
-    const double scale = iRn * s.irho;
-    double eta = s.g3 * coef1 * coef2 + value1 - s.mu2 * coef3;
-    double om = sqrt(fmax(eta, 1.0e-6 * kDblEps)) * scale;
-    const double ca = k.calpha * iRn;
-    for (int it = 0; it < 3; it++) {
-        const cd ups = fast_upsilon(sqrt(ca * om), s.gam, s.k3);
-        const double ur = ups.re * coef1, ui = ups.im * coef1;
-        const double mu_eff = s.mu + ui * PGn / (4.0 * om);
-        eta = ur * coef2 + value1 - mu_eff * mu_eff * coef3;
-        om = sqrt(fmax(eta, 1.0e-6 * kDblEps)) * scale;
-    }
-    const cd Lp = fast_lp(sqrt(ca * om), Rn);
-    // solvePG
-    const double Rbar = 0.5 * (Rp + Rn);
-    const double iRbar = 1.0 / Rbar;
-    const double Rx_ = Rbar * k.iR0, Rpx = Rp * k.iR0, Rnx = Rn * k.iR0;
-    const double iL = 1.0 / (Lp.re * Lp.re + Lp.im * Lp.im);
-    const double Lnr = Lp.re * iL, Lni = Lp.im * iL;
-    const double iom = 1.0 / om;
-    const double cdt = s.C0 * iRbar * k.iPG0;   // c0 / dt
-    const double c0 = dtL * cdt;
-    const double c3 = 1.5 * s.gam * (Rp - Rn) * iRbar + c0 * Lnr * 0.5 * Rx_ * Rx_ * Rx_;
-    const double q = cdt * Lni * iom;           // c0 Lp_NI / (omega dt)
-    const double c1 = 1.0 + c3 - q * Rpx * Rpx * Rpx;
-    const double c2 = 1.0 - c3 - q * Rnx * Rnx * Rnx;
-    const double PGp = (c2 * PGn + c0 * Lnr * k.PG0) / c1;
-    L.PG = PGp;
-    // heat and viscous dissipation
-    const double s1 = iL * k.T0_PG0R03;
-    const double dTdr = Lp.re * s1 * (k.PG0R03 - 0.5 * (PGp + PGn) * Rbar * Rbar * Rbar) +
-                        Lp.im * s1 * (PGp * Rp * Rp * Rp - PGn * Rn * Rn * Rn) * iom / dtL;
-    L.SumHeat -= s.four_pi_K0 * Rp * Rp * dTdr * dtL;
-    L.SumVis += s.sixteen_pi_mu * Rp * d1R * d1R * dtL;
-    L.counter++;
-}
 
 }  // namespace brp

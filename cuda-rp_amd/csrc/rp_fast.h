@@ -1,0 +1,286 @@
+// rp_fast.h -- the RP sub-step with a short dependency chain, generic over the
+// scalar type.
+//
+// Same model as rp_device.h (bubble_CUDA_kernel.cuh:227-340), rearranged for
+// latency, since one collapsing bubble's thousands of sequential sub-steps
+// bound a whole launch:
+//   * alpha = s (1 + i) always (rp_alpha), so
+//       coth(alpha) = (1 - |E|^2 + 2 i Im E) / |1 - E|^2,  E = exp(-2 alpha),
+//     from expm1(-2s) and sincos(s); for s > 20, |E| < 5e-18 and
+//     alpha coth(alpha) - 1 = (s - 1, s) to within a relative 1e-17, so the
+//     transcendentals are skipped (this is where collapsing bubbles spend
+//     about half their sub-steps);
+//   * 1 / alpha^2 = -i / (2 s^2); Lp = R (1/u + 1.5 i / s^2),
+//     u = alpha coth(alpha) - 1;
+//   * omega_N and 1/omega_N from one rsqrt of eta; sqrt(c_alpha omega / R)
+//     and its reciprocal from one rsqrt;
+//   * reciprocals by v_rcp_f64 and two Newton steps (drcp), step-size bounds
+//     by rsqrt; every fused multiply-add is written out (the file is compiled
+//     with contraction off), so the instruction sequence is fixed by the
+//     source.
+// T = double is the forward.  T = dd (value, tangent) is the adjoint's
+// linearisation: every dd operation computes its value with the same single
+// operation the double code uses, so the value part -- and with it the
+// sub-step schedule -- is bit-identical to the forward's, and the tangent
+// follows the same branches.
+#pragma once
+
+#include <type_traits>
+
+#include "rp_device.h"
+
+namespace brp {
+
+// Reciprocal of a normal double: hardware estimate + two Newton steps.
+__device__ inline double drcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+struct dd {
+    double v, d;  // value, tangent
+    __device__ dd() = default;
+    __device__ dd(double x) : v(x), d(0.0) {}
+    __device__ dd(double x, double t) : v(x), d(t) {}
+};
+
+__device__ inline dd operator+(dd a, dd b) { return dd(a.v + b.v, a.d + b.d); }
+__device__ inline dd operator-(dd a, dd b) { return dd(a.v - b.v, a.d - b.d); }
+__device__ inline dd operator-(dd a) { return dd(-a.v, -a.d); }
+__device__ inline dd operator*(dd a, dd b) { return dd(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ inline dd operator*(double a, dd b) { return dd(a * b.v, a * b.d); }
+__device__ inline dd operator*(dd a, double b) { return dd(a.v * b, a.d * b); }
+__device__ inline dd operator/(dd a, double b) { return dd(a.v / b, a.d / b); }
+
+__device__ inline double val(double x) { return x; }
+__device__ inline double val(dd x) { return x.v; }
+__device__ inline double fma_(double a, double b, double c) { return fma(a, b, c); }
+__device__ inline dd fma_(dd a, dd b, dd c) {
+    return dd(fma(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d);
+}
+__device__ inline double rcp_(double x) { return drcp(x); }
+__device__ inline dd rcp_(dd x) {
+    const double r = drcp(x.v);
+    return dd(r, -r * r * x.d);
+}
+__device__ inline double rsqrt_(double x) { return rsqrt(x); }
+__device__ inline dd rsqrt_(dd x) {
+    const double r = rsqrt(x.v);
+    return dd(r, -0.5 * r * r * r * x.d);
+}
+__device__ inline double expm1_(double x) { return expm1(x); }
+__device__ inline dd expm1_(dd x) {
+    const double m = expm1(x.v);
+    return dd(m, (m + 1.0) * x.d);
+}
+__device__ inline void sincos_(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ inline void sincos_(dd x, dd *s, dd *c) {
+    double sn, cn;
+    sincos(x.v, &sn, &cn);
+    *s = dd(sn, cn * x.d);
+    *c = dd(cn, -sn * x.d);
+}
+
+template <typename T>
+struct cx {
+    T re, im;
+};
+
+// alpha coth(alpha) - 1 for alpha = s (1 + i), s > 0.
+template <typename T>
+__device__ inline cx<T> alpha_coth_m1_g(T s) {
+    if (val(s) > 20.0) return cx<T>{s - T(1.0), s};
+    const T m = expm1_(-2.0 * s);  // exp(-2s) - 1
+    T sn, cn;
+    sincos_(s, &sn, &cn);
+    const T e2 = T(1.0) + m;
+    const T s2 = 2.0 * (sn * cn);              // sin 2s
+    const T c2 = fma_(-2.0 * sn, sn, T(1.0));  // cos 2s
+    const T ei = -(e2 * s2);                   // Im E
+    const T one_m_er = fma_(2.0 * sn, sn, -(m * c2));  // 1 - Re E
+    const T iD = rcp_(fma_(one_m_er, one_m_er, ei * ei));
+    const T cr = -(m * (m + T(2.0))) * iD;     // (1 - |E|^2) / |1 - E|^2
+    const T ci = (2.0 * ei) * iD;
+    return cx<T>{fma_(s, cr - ci, T(-1.0)), s * (cr + ci)};
+}
+
+// Upsilon(alpha) (bubble_CUDA_kernel.cuh:296-311) from s and 1/s.
+template <typename T>
+__device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3) {
+    T tr, ti;  // (alpha coth alpha - 1) / alpha^2
+    if (val(s) * 1.4142135623730951 > 1.0e-1) {
+        const cx<T> u = alpha_coth_m1_g(s);
+        const T h = 0.5 * (is * is);  // 1 / (2 s^2)
+        tr = u.im * h;
+        ti = -(u.re * h);
+    } else {  // 1/3 + a^2 (-1/45 + a^2 (2/945 - a^2/4725)),  a^2 = i q
+        const T q = 2.0 * (s * s);
+        T ur = T(2.0 / 945.0), ui = -(q * (1.0 / 4725.0));
+        T nr = fma_(-q, ui, T(-1.0 / 45.0)), ni = q * ur;
+        tr = fma_(-q, ni, T(1.0 / 3.0));
+        ti = q * nr;
+    }
+    const T dr = fma_(T(k3), tr, T(1.0)), di = k3 * ti;
+    const T f = (3.0 * gam) * rcp_(fma_(dr, dr, di * di));
+    return cx<T>{dr * f, -(di * f)};
+}
+
+// solveLp (bubble_CUDA_kernel.cuh:314-320) from s and 1/s.
+template <typename T>
+__device__ inline cx<T> lp_g(T s, T is, T R) {
+    if (val(s) * 1.4142135623730951 > 1.0e-1) {
+        const cx<T> u = alpha_coth_m1_g(s);
+        const T iu = rcp_(fma_(u.re, u.re, u.im * u.im));
+        return cx<T>{R * (u.re * iu), R * fma_(1.5 * is, is, -(u.im * iu))};
+    }
+    // 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875)),  a^2 = i q
+    const T q = 2.0 * (s * s);
+    T ur = T(2.0 / 7875.0), ui = -(q * (37.0 / 3031875.0));
+    T nr = fma_(-q, ui, T(-1.0 / 175.0)), ni = q * ur;
+    return cx<T>{R * fma_(-q, ni, T(1.0 / 5.0)), R * (q * nr)};
+}
+
+// omega = sqrt(max(eta, floor)) / (rho R) and 1 / omega from one rsqrt.
+template <typename T>
+__device__ inline void omega_g(T eta, T scale, T iscale, T *om, T *iom) {
+    const T e = val(eta) >= 1.0e-6 * kDblEps ? eta : T(1.0e-6 * kDblEps);
+    const T r = rsqrt_(e);
+    *om = (e * r) * scale;
+    *iom = r * iscale;
+}
+
+template <typename T>
+struct RPLaneF {
+    T Rp, Rn, d1R, PG, Rt, PC0, PC1, PC2;
+    double dtL, remain, time, re_n, SumHeat, SumVis;
+    int counter;
+};
+
+// Entry of BubbleRadiusKernel (bubble_CUDA_kernel.cuh:487-504).
+template <typename T>
+__device__ inline void rp_begin_g(RPLaneF<T> &L, T Rp, T Rn, T d1R, T PG, double dt_n, double re_n,
+                                  T PLm, T PLn, T PLp, double dt_mix, double PL0) {
+    L.Rp = Rp;
+    L.Rn = Rn;
+    L.d1R = d1R;
+    L.PG = PG;
+    L.dtL = dt_n;
+    L.re_n = re_n;
+    L.remain = re_n + dt_mix;
+    L.time = -re_n;
+    L.PC0 = PLn + T(PL0);
+    L.PC1 = (0.5 * (PLp - PLm)) / dt_mix;
+    L.PC2 = (0.5 * (PLp + PLm - 2.0 * PLn)) / (dt_mix * dt_mix);
+    L.SumHeat = 0.0;
+    L.SumVis = 0.0;
+    L.Rt = T(0.0);
+    L.counter = 0;
+}
+
+// One pass of the sub-step loop body (bubble_CUDA_kernel.cuh:508-538).
+template <typename T>
+__device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const RPFastShared &s) {
+    // solveRayleighPlesset (bubble_CUDA_kernel.cuh:227-260)
+    const T PGn = L.PG;
+    const double t = L.time;
+    const T PL = fma_(L.PC2, T(t * t), fma_(L.PC1, T(t), L.PC0));
+    const T Rm = L.Rn;
+    const T Rn = L.Rp;
+    const T iRn = rcp_(Rn);
+    T d1R = L.d1R;
+    const T sv = iRn * fma_(T(s.four_mu), d1R, T(s.two_sig));  // (2 sig + 4 mu dR) / R
+    const T Rnd2R = fma_(-1.5 * d1R, d1R, (PGn - PL - sv) * s.irho);
+    const double dtm = L.dtL;
+    double dtp = fmin(1.01 * dtm, s.dt0);
+    const double ad = fabs(val(d1R)), aa = fabs(val(Rnd2R));
+    if (ad > kDblEps) dtp = fmin(dtp, (0.01 * val(Rn)) * drcp(ad));
+    if (aa > kDblEps) dtp = fmin(dtp, (0.1 * val(Rn)) * rsqrt(aa));
+    const double idtm = drcp(dtm);
+    const double r = dtp * idtm;
+    const T acc = (dtp * Rnd2R) * iRn;  // (dtp / Rn) Rnd2R
+    const T Rp = fma_(acc, T(0.5 * (dtp + dtm)), fma_(T(1.0 + r), Rn, -r * Rm));
+    const double dm2 = dtm * dtm, dp2 = dtp * dtp;
+    const double iden = drcp(dtm * dtp * (dtm + dtp));
+    d1R = fma_(fma_(T(dm2), Rp, fma_(T(dp2 - dm2), Rn, -dp2 * Rm)), T(iden), acc);
+    const double dtL = dtp;
+    if (dtp >= L.remain) {  // last sub-step: R_t at the sync time
+        const double d = L.remain;
+        const double r2 = d * idtm;
+        L.Rt = fma_((d * Rnd2R) * iRn, T(0.5 * (d + dtm)), fma_(T(1.0 + r2), Rn, -r2 * Rm));
+        dtp = d;
+    } else {
+        L.Rt = Rp;
+    }
+    L.remain -= dtp;
+    L.time = t + dtL;
+    L.dtL = dtL;
+    L.Rp = Rp;
+    L.Rn = Rn;
+    L.d1R = d1R;
+    // solveOmegaN at (PGn, Rn) (bubble_CUDA_kernel.cuh:263-289)
+    const T Rx = Rn * k.iR0;
+    const T coef1 = k.PG0 * rcp_(PGn * (Rx * Rx * Rx));
+    const T coef2 = PGn * s.rho;
+    const T coef3 = 4.0 * (iRn * iRn);
+    const T value1 = s.m2sig_rho * iRn;
+    const T scale = iRn * s.irho;
+    const T iscale = Rn * s.rho;
+    T eta = fma_(s.g3 * coef1, coef2, fma_(T(-s.mu2), coef3, value1));
+    T om, iom;
+    omega_g(eta, scale, iscale, &om, &iom);
+    const T ca = k.calpha * iRn;
+    for (int it = 0; it < 3; it++) {
+        const T x = ca * om;
+        const T isa = rsqrt_(x);
+        const cx<T> ups = upsilon_g(x * isa, isa, s.gam, s.k3);
+        const T ur = ups.re * coef1, ui = ups.im * coef1;
+        const T mu_eff = fma_(ui * PGn, 0.25 * iom, T(s.mu));  // mu + ui PG / (4 omega)
+        eta = fma_(ur, coef2, fma_(-(mu_eff * mu_eff), coef3, value1));
+        omega_g(eta, scale, iscale, &om, &iom);
+    }
+    const T x = ca * om;
+    const T isa = rsqrt_(x);
+    const cx<T> Lp = lp_g(x * isa, isa, Rn);
+    // solvePG (bubble_CUDA_kernel.cuh:322-340)
+    const T Rbar = 0.5 * (Rp + Rn);
+    const T iRbar = rcp_(Rbar);
+    const T Rx_ = Rbar * k.iR0, Rpx = Rp * k.iR0, Rnx = Rn * k.iR0;
+    const T iL = rcp_(fma_(Lp.re, Lp.re, Lp.im * Lp.im));
+    const T Lnr = Lp.re * iL, Lni = Lp.im * iL;
+    const T cdt = (s.C0 * iRbar) * k.iPG0;  // c0 / dt
+    const T c0 = dtL * cdt;
+    const T c3 = fma_(c0 * Lnr, 0.5 * (Rx_ * Rx_ * Rx_), ((1.5 * s.gam) * (Rp - Rn)) * iRbar);
+    const T q = (cdt * Lni) * iom;  // c0 Lp_NI / (omega dt)
+    const T c1 = fma_(-q, Rpx * Rpx * Rpx, T(1.0) + c3);
+    const T c2 = fma_(-q, Rnx * Rnx * Rnx, T(1.0) - c3);
+    const T PGp = fma_(c2, PGn, (c0 * Lnr) * k.PG0) * rcp_(c1);
+    L.PG = PGp;
+    if constexpr (std::is_same<T, double>::value) {
+        // heat and viscous dissipation (Q_B)
+        const double s1 = iL * k.T0_PG0R03;
+        const double dTdr =
+            Lp.re * s1 * (k.PG0R03 - 0.5 * (PGp + PGn) * Rbar * Rbar * Rbar) +
+            Lp.im * s1 * (PGp * Rp * Rp * Rp - PGn * Rn * Rn * Rn) * iom / dtL;
+        L.SumHeat -= s.four_pi_K0 * Rp * Rp * dTdr * dtL;
+        L.SumVis += s.sixteen_pi_mu * Rp * d1R * d1R * dtL;
+    }
+    L.counter++;
+}
+
+// Exit of BubbleRadiusKernel (bubble_CUDA_kernel.cuh:542-550).
+__device__ inline void rp_end_g(const RPLaneF<double> &L, double dt_mix, RPState *out, double *R_t,
+                                double *Q_B) {
+    out->Rp = L.Rp;
+    out->Rn = L.Rn;
+    out->d1R = L.d1R;
+    out->PG = L.PG;
+    out->dt = L.dtL;
+    out->re = L.remain;
+    *R_t = L.Rt;
+    *Q_B = (L.SumHeat + L.SumVis) / (dt_mix - L.remain + L.re_n);
+}
+
+}  // namespace brp

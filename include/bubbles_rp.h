@@ -263,7 +263,8 @@ typedef struct brp_adjoint_stats {
     double reverse_ms;     /* recomputation + adjoint kernels, host wall time */
     int64_t memory_bytes;  /* device bytes of checkpoints, tape and adjoint state */
     int32_t n_checkpoints, segment;
-    int64_t pad_;
+    int64_t schedule_mismatch; /* replayed RP calls whose sub-step count differed
+                                  from the forward's (0 when the replay is exact) */
 } brp_adjoint_stats;
 
 /* Discrete adjoint of n_steps coupled steps from the current state (the

@@ -47,6 +47,7 @@ def test_adjoint_matches_frozen_schedule_differences(brp):
     g, r = _gpu_adjoint(brp, cfg, bub, (amp, phase))
     assert r["J"] == pytest.approx(J0, rel=1e-9)
     assert r["n_checkpoints"] == 5
+    assert r["schedule_mismatch"] == 0  # the replay took the forward's sub-steps
     worst = 0.0
     for e in range(len(amp)):
         for which, base in (("amp", amp[e]), ("phase", 1.0)):
