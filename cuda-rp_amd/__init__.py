@@ -20,7 +20,9 @@ from . import abi
 from .abi import *  # noqa: F401,F403  (re-export the ABI mirror)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libbubbles_rp.so")
+# BRP_LIBRARY selects another build of the same library (e.g. a variant built
+# with different compile-time options); by default the in-tree build.
+LIB_PATH = os.environ.get("BRP_LIBRARY") or os.path.join(_HERE, "lib", "libbubbles_rp.so")
 _lib = None
 
 

@@ -27,7 +27,7 @@ FIELD_NAMES = {"p0": FIELD_P0, "px": FIELD_PX, "py": FIELD_PY, "vx": FIELD_VX,
                "rho_m": FIELD_RHO_M, "T": FIELD_T}
 
 BUB_NAMES = ["x", "y", "R_t", "R_p", "R_n", "d1R", "PG", "PL_p", "PL_n", "PL_m", "Q_B",
-             "n_B", "dt", "re", "vx", "vy", "ibm_x", "ibm_y", "ibn_x", "ibn_y", "R0"]
+             "n_B", "dt", "re", "vx", "vy", "ibm_x", "ibm_y", "ibn_x", "ibn_y", "R0", "sub"]
 BUB = {n: k for k, n in enumerate(BUB_NAMES)}
 
 

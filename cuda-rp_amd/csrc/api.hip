@@ -44,6 +44,11 @@ struct Snapshot {
 struct StepRecord {
     void *p0_start = nullptr;
     double *x_start = nullptr, *y_start = nullptr;
+    int64_t *fgC = nullptr;                 // f_g at the start (the step's f_gn)
+    double *rp[6] = {};                     // RP start state R, Rn, d1R, PG, dt, re
+    double *PLm = nullptr, *PLn = nullptr;  // liquid-pressure history of the step
+    void *vx_end = nullptr, *vy_end = nullptr;   // velocities after the update
+    double *x_end = nullptr, *y_end = nullptr;   // positions after the motion
     std::vector<void *> p0_it;       // [k] after evaluation k (k = 0..n_pic)
     std::vector<int64_t *> fg_it;    // [k] void-fraction iterate k
     std::vector<double *> PLp_it;    // [k] gathered PL_p of iteration k >= 1
@@ -292,7 +297,8 @@ int alloc_bubbles(brp_handle *h, int n) {
         if ((rc = dalloc(h, (void **)a, bytes))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.sub, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.order, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
-    if ((rc = dalloc(h, (void **)&h->b.part, sizeof(int) * (size_t)((n + 255) / 256 + 2)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->b.part, sizeof(int) * 2 * (size_t)((n + 255) / 256 + 1))))
+        return rc;
     if ((rc = dalloc(h, (void **)&h->b.bbox_blk, sizeof(int) * 4 * (size_t)((n + 255) / 256 + 1))))
         return rc;
     return BRP_OK;
@@ -513,13 +519,23 @@ int rec_grow(brp_handle *h, StepRecord *r, int k) {
     };
     int rc;
     if (!r->p0_start) {
-        void *a, *b, *c;
-        if ((rc = get(fb, &a)) || (rc = get(sizeof(double) * std::max(h->nb, 1), &b)) ||
-            (rc = get(sizeof(double) * std::max(h->nb, 1), &c)))
-            return rc;
-        r->p0_start = a;
-        r->x_start = (double *)b;
-        r->y_start = (double *)c;
+        const size_t nbb = sizeof(double) * std::max(h->nb, 1);
+        void *q[16];
+        const size_t sz[16] = {fb,  nbb, nbb, sizeof(int64_t) * h->frame,
+                               nbb, nbb, nbb, nbb, nbb, nbb, nbb, nbb, fb, fb, nbb, nbb};
+        for (int k = 0; k < 16; k++)
+            if ((rc = get(sz[k], &q[k]))) return rc;
+        r->p0_start = q[0];
+        r->x_start = (double *)q[1];
+        r->y_start = (double *)q[2];
+        r->fgC = (int64_t *)q[3];
+        for (int k = 0; k < 6; k++) r->rp[k] = (double *)q[4 + k];
+        r->PLm = (double *)q[10];
+        r->PLn = (double *)q[11];
+        r->vx_end = q[12];
+        r->vy_end = q[13];
+        r->x_end = (double *)q[14];
+        r->y_end = (double *)q[15];
     }
     while ((int)r->p0_it.size() < k) {
         void *a, *b, *c = nullptr, *d = nullptr, *e = nullptr;
@@ -582,9 +598,16 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
             int rc = rec_grow(h, rec, 1);
             if (rc) return rc;
             HIPCHK(h, hipMemcpyAsync(rec->p0_start, h->p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+            HIPCHK(h, hipMemcpyAsync(rec->fgC, h->fgb[old_cur], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
             if (h->nb > 0) {
-                HIPCHK(h, hipMemcpyAsync(rec->x_start, h->b.x, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
-                HIPCHK(h, hipMemcpyAsync(rec->y_start, h->b.y, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+                const size_t nbb = sizeof(double) * h->nb;
+                const int c = h->rp_cur;
+                double *src[10] = {h->b.x, h->b.y, h->b.R[c], h->b.Rn[c], h->b.d1R[c], h->b.PG[c],
+                                   h->b.bdt[c], h->b.re[c], h->b.PL[h->pl_m], h->b.PL[h->pl_n]};
+                double *dst[10] = {rec->x_start, rec->y_start, rec->rp[0], rec->rp[1], rec->rp[2],
+                                   rec->rp[3], rec->rp[4], rec->rp[5], rec->PLm, rec->PLn};
+                for (int q = 0; q < 10; q++)
+                    HIPCHK(h, hipMemcpyAsync(dst[q], src[q], nbb, hipMemcpyDeviceToDevice, s));
             }
             rec->first = h->first_step;
             rec->cur = h->rp_cur;
@@ -636,7 +659,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                     if (rc) return rc;
                 }
                 HIPCHK(h, hipEventRecord(h->ev[7], s));
-                launch_order_partition(s, h->b, h->bp.heavy_threshold);
+                launch_order_partition(s, h->b, h->bp.heavy_threshold, h->bp.extreme_threshold);
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
                 launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
                                       p0, h->fgb[target], h->sc_d, loop);
@@ -706,7 +729,15 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         st.time = h->t;
         st.resimax = resimax;
         st.n_pic = loop;
-        if (rec) rec->n_pic = loop;
+        if (rec) {
+            rec->n_pic = loop;
+            HIPCHK(h, hipMemcpyAsync(rec->vx_end, vx, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+            HIPCHK(h, hipMemcpyAsync(rec->vy_end, vy, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+            if (h->nb > 0) {
+                HIPCHK(h, hipMemcpyAsync(rec->x_end, h->b.x, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+                HIPCHK(h, hipMemcpyAsync(rec->y_end, h->b.y, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
+            }
+        }
         if (ctl && ctl->npic_out) ctl->npic_out->push_back(loop);
         st.rp_max_substeps = h->sc_h->sub_max;
         st.rp_mean_substeps = (loop > 0 && h->nb > 0)
@@ -761,10 +792,23 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, co
     const GridParams &g = h->g;
     const bool bub = h->cfg.bub_enable && h->nb > 0;
     const int n = rec.n_pic;
-    const int64_t *gC = h->fgb[rec.f_cur];
+    const int64_t *gC = rec.fgC;
     const F *ax = (const F *)h->ax, *bx = (const F *)h->bx, *ay = (const F *)h->ay,
             *by = (const F *)h->by;
-    if (seed) launch_adj_seed<F>(s, g, (const F *)h->p0, a.f.p0, box, cJ);
+    // the bubble arrays as the step saw them: positions after the motion, RP
+    // start state in slot 0, PL history in slots 0 (m) and 1 (n)
+    BubbleArrays bv = h->b;
+    bv.x = rec.x_end;
+    bv.y = rec.y_end;
+    bv.R[0] = rec.rp[0];
+    bv.Rn[0] = rec.rp[1];
+    bv.d1R[0] = rec.rp[2];
+    bv.PG[0] = rec.rp[3];
+    bv.bdt[0] = rec.rp[4];
+    bv.re[0] = rec.rp[5];
+    bv.PL[0] = rec.PLm;
+    bv.PL[1] = rec.PLn;
+    if (seed) launch_adj_seed<F>(s, g, (const F *)rec.p0_it[n], a.f.p0, box, cJ);
     // start-of-step f_g adjoints: f_gn' = f_g and f_gm' = f_gn of the start
     double *LgC = a.LfN, *LfNs = a.LfM, *LfMs = a.LfC;
     double *const none[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -772,19 +816,18 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, co
         launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
         launch_adj_pressure<F>(s, g, false, a.f, rec.fg_it[k], gC, k == n ? a.LfC : nullptr, a.Lf,
                                LgC, nullptr, nullptr, ax, bx, ay, by);
-        launch_adj_scatter(s, g, h->bp, h->b, rec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
-        launch_adj_rp(s, g, h->bp, h->b, rec.cur, rec.pl_m, rec.pl_n, rec.PLp_it[k], a.lRt,
-                      k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn, a.lPLm, rec.sub_it[k],
-                      a.mismatch);
-        launch_adj_gather<F>(s, g, h->bp, h->b, (const F *)rec.p0_it[k - 1], a.lPLp,
+        launch_adj_scatter(s, g, h->bp, bv, rec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
+        launch_adj_rp(s, g, h->bp, bv, 0, 0, 1, rec.PLp_it[k], a.lRt, k == n ? a.LR : none,
+                      a.LRs, a.lPLp, a.lPLn, a.lPLm, rec.sub_it[k], a.mismatch);
+        launch_adj_gather<F>(s, g, h->bp, bv, (const F *)rec.p0_it[k - 1], a.lPLp,
                              k == n ? a.LA : nullptr, a.f.p0, a.Lx, a.Ly);
     }
     launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
     launch_adj_pressure<F>(s, g, true, a.f, rec.fg_it[0], gC, n == 0 ? a.LfC : nullptr, nullptr,
                            LgC, LfNs, LfMs, ax, bx, ay, by);
     if (bub)
-        launch_adj_motion<F>(s, g, h->bp, h->nb, rec.x_start, rec.y_start, (const F *)h->vx,
-                             (const F *)h->vy, a.f.vx, a.f.vy, a.Lx, a.Ly);
+        launch_adj_motion<F>(s, g, h->bp, h->nb, rec.x_start, rec.y_start, (const F *)rec.vx_end,
+                             (const F *)rec.vy_end, a.f.vx, a.f.vy, a.Lx, a.Ly);
     launch_adj_velocity_ghosts(s, g, a.f.vx, a.f.vy);
     launch_adj_velocity<F>(s, g, a.f.vx, a.f.vy, (const F *)rec.p0_start, gC, rec.first,
                            (const F *)h->cxa, (const F *)h->cxb, (const F *)h->cya,
@@ -819,20 +862,19 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         1.0 / ((double)ob.window * (box[1] - box[0] + 1) * (double)(box[3] - box[2] + 1));
     hipStream_t s = h->stream;
     AdjMem mem;
-    std::vector<Snapshot> ck(nseg), tape(K);
+    std::vector<Snapshot> ck(nseg);
     Snapshot fin;
-    StepRecord rec;
+    std::vector<StepRecord> recs(K);
     struct Cleanup {
-        std::vector<Snapshot> *a, *b;
+        std::vector<Snapshot> *a;
         Snapshot *c;
-        StepRecord *r;
+        std::vector<StepRecord> *r;
         ~Cleanup() {
             for (auto &x : *a) snap_free(x);
-            for (auto &x : *b) snap_free(x);
             snap_free(*c);
-            rec_free(*r);
+            for (auto &x : *r) rec_free(x);
         }
-    } cleanup{&ck, &tape, &fin, &rec};
+    } cleanup{&ck, &fin, &recs};
     int rc;
     double *jacc = nullptr;
     if ((rc = mem.get(h, &jacc, (size_t)n))) return rc;
@@ -874,30 +916,22 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         if ((rc = mem.get(h, q, nb))) return rc;
     if ((rc = mem.get(h, &a.mismatch, 1))) return rc;
 
-    // reverse sweep: per segment, a tape of step start states, then each step
-    // replayed with its intermediates recorded and reversed
+    // reverse sweep: per segment, the steps are recomputed from the segment's
+    // checkpoint with their intermediates recorded, then reversed
     for (int sg = nseg - 1; sg >= 0; sg--) {
         if ((rc = snap_restore(h, ck[sg]))) return rc;
         const int s0 = sg * K + 1, s1 = std::min(n, (sg + 1) * K);
         for (int st = s0; st <= s1; st++) {
-            if ((rc = snap_save(h, tape[st - s0]))) return rc;
-            if (st < s1) {
-                StepCtl c;
-                c.force_npic = &npic[st - 1];
-                if ((rc = run_steps<F>(h, 1, nullptr, &c))) return rc;
-            }
-        }
-        for (int st = s1; st >= s0; st--) {
-            if (st < s1 && (rc = snap_restore(h, tape[st - s0]))) return rc;
             StepCtl c;
             c.force_npic = &npic[st - 1];
-            c.rec = &rec;
+            c.rec = &recs[st - s0];
             if ((rc = run_steps<F>(h, 1, nullptr, &c))) return rc;
-            if (rec.n_pic != npic[st - 1])
+            if (recs[st - s0].n_pic != npic[st - 1])
                 return set_err(h, BRP_ERR_STATE, "replay of step %d took %d Picard iterations, not %d",
-                               st, rec.n_pic, npic[st - 1]);
-            adjoint_step<F>(h, rec, st > n - ob.window, cJ, box, a);
+                               st, recs[st - s0].n_pic, npic[st - 1]);
         }
+        for (int st = s1; st >= s0; st--)
+            adjoint_step<F>(h, recs[st - s0], st > n - ob.window, cJ, box, a);
     }
     HIPCHK(h, hipStreamSynchronize(s));
     const auto t2 = clk::now();
@@ -935,9 +969,14 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         stats->J = J;
         stats->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         stats->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
-        // step record: p0 and f_g per evaluation, PL_p and R_t per iteration
-        const size_t recb = rec.p0_it.size() * ((h->fsz + 8) * fr + 16 * nb) + (h->fsz * fr + 16 * nb);
-        stats->memory_bytes = (int64_t)((nseg + K + 1) * snap_bytes(h) + mem.bytes + recb);
+        // step records: per evaluation p0 and f_g, per iteration PL_p, R_t and
+        // sub-step counts, per step start/end fields and bubble state
+        size_t recb = 0;
+        for (const StepRecord &r : recs)
+            if (r.p0_start)
+                recb += r.p0_it.size() * ((h->fsz + 8) * fr + 20 * nb) +
+                        (3 * h->fsz + 8) * fr + 14 * 8 * nb;
+        stats->memory_bytes = (int64_t)((nseg + 1) * snap_bytes(h) + mem.bytes + recb);
         stats->n_checkpoints = nseg;
         stats->segment = K;
     }
@@ -1087,6 +1126,7 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     bp.cylindrical = c.tr_cylindrical;
     bp.rp_exact = c.rp_exact ? 1 : 0;
     bp.heavy_threshold = 48;
+    bp.extreme_threshold = 256;
 
     int rc = h->prec == BRP_PREC_FP64 ? alloc_grid<double>(h) : alloc_grid<float>(h);
     if (rc) return fail(rc);
@@ -1357,6 +1397,12 @@ int brp_get_bubbles(brp_handle *h, int32_t which, double *dst) {
         case BRP_BUB_VX: src = h->b.vx; break;
         case BRP_BUB_VY: src = h->b.vy; break;
         case BRP_BUB_R0: src = h->b.R0; break;
+        case BRP_BUB_SUB: {
+            std::vector<int> iv(n);
+            HIPCHK(h, hipMemcpy(iv.data(), h->b.sub, sizeof(int) * n, hipMemcpyDeviceToHost));
+            for (int k = 0; k < n; k++) dst[h->perm[k]] = (double)iv[k];
+            return BRP_OK;
+        }
         default: {
             HIPCHK(h, hipMalloc((void **)&tmpd, sizeof(double) * n));
             launch_bubble_index(h->stream, h->g, h->b, tmpd, which - BRP_BUB_IBM_X);

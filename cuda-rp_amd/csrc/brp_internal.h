@@ -77,7 +77,7 @@ struct BubbleArrays {
     double *vx, *vy;      // v_B (= v_L)
     int *sub;             // sub-steps of the last RP call, per bubble
     int *order;           // work-queue order: bubbles with many sub-steps first
-    int *part;            // partition scratch: per-block counts, then the heavy total
+    int *part;            // partition scratch: per-block extreme and heavy offsets + totals
     int *bbox_blk;        // per-block stencil bounding boxes of k_motion (4 ints each)
 };
 
@@ -91,6 +91,7 @@ struct BubbleParams {
     int cylindrical;
     int rp_exact;
     int heavy_threshold;    // sub-steps that put a bubble at the queue front
+    int extreme_threshold;  // sub-steps that give a bubble a wavefront of its own
 };
 
 }  // namespace brp
@@ -126,7 +127,7 @@ void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &
                         const BubbleArrays &b, int cur, int plm, int pln, int plp,
                         const F *p0, int64_t *fg_target, StepScalars *sc, int slot);
 int bubble_step_blocks(int n_bubbles);
-void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold);
+void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);

@@ -284,7 +284,16 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
                           4.0 * bp.pi * bp.K0,
                           16.0 * bp.pi * bp.mu};
     RPFastConst fk{0, 0, 0, 0, 0, 0, 0};
+    // extreme bubbles (the queue's first E entries) run one per workgroup in
+    // lane 0 of its first wavefront, so that their long sub-step chains do not
+    // share a wavefront with each other; the shared queue starts after them
+    const int E = min(b.part[(b.n + 255) / 256], (int)gridDim.x);
     int idx = -1;                  // bubble of this lane, -1 when idle
+    bool pre = false;
+    if (threadIdx.x == 0 && (int)blockIdx.x < E) {
+        idx = b.order[blockIdx.x];
+        pre = true;
+    }
     int batch_next = 0, batch_end = 0;  // wave-uniform
     bool exhausted = false;             // wave-uniform
     RPLane L;              // operation-by-operation path
@@ -299,19 +308,20 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
         // ---- refill idle lanes from the wave's batch / the global queue
         bool need = idx < 0;
         unsigned long long m = __ballot(need);
-        bool fresh = false;
+        bool fresh = pre;
+        pre = false;
         while (m != 0ull && !exhausted) {
             if (batch_next >= batch_end) {
                 const int leader = __ffsll((long long)m) - 1;
                 unsigned int base = 0;
                 if (lane == leader) base = atomicAdd(&sc->work[slot], 64u);
                 base = __shfl(base, leader, 64);
-                if ((int)base >= b.n) {
+                if ((int)base >= b.n - E) {
                     exhausted = true;
                     break;
                 }
-                batch_next = (int)base;
-                batch_end = min((int)base + 64, b.n);
+                batch_next = (int)base + E;
+                batch_end = min((int)base + E + 64, b.n);
             }
             const int avail = batch_end - batch_next;
             const int rank = __popcll(m & lanemask_lt(lane));
@@ -489,11 +499,19 @@ void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &
 // sequential sub-step chains start at once and overlap everything else),
 // each group in index (cell-sorted) order.  Three passes: per-block counts,
 // one-block exclusive scan, scatter.
-__global__ __launch_bounds__(256) void k_part_count(const int *sub, int n, int thr, int *cnt) {
+// Work-queue order: bubbles whose last RP call took >= ext sub-steps first
+// ("extreme": each gets a wavefront of its own at launch), then those with
+// >= thr ("heavy"), then the rest; stable within each class.
+__global__ __launch_bounds__(256) void k_part_count(const int *sub, int n, int ext, int thr,
+                                                    int *cntE, int *cntH) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
-    const int h = (idx < n && sub[idx] >= thr) ? 1 : 0;
-    const int c = __syncthreads_count(h);
-    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+    const int v = idx < n ? sub[idx] : 0;
+    const int ce = __syncthreads_count(idx < n && v >= ext);
+    const int ch = __syncthreads_count(idx < n && v >= thr && v < ext);
+    if (threadIdx.x == 0) {
+        cntE[blockIdx.x] = ce;
+        cntH[blockIdx.x] = ch;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_part_scan(int *cnt, int nblk) {
@@ -530,36 +548,55 @@ __global__ __launch_bounds__(1024) void k_part_scan(int *cnt, int nblk) {
     if (threadIdx.x == 0) cnt[nblk] = carry;
 }
 
-__global__ __launch_bounds__(256) void k_part_scatter(const int *sub, int n, int thr,
-                                                       const int *off, int nblk, int *order) {
+__device__ inline int block_rank(bool f, int *wcnt, int *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(f);
+    __syncthreads();
+    if (lane == 0) wcnt[w] = __popcll(m);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int q = 0; q < 4; q++) {
+        if (q < w) before += wcnt[q];
+        all += wcnt[q];
+    }
+    *total = all;
+    return before + __popcll(m & lanemask_lt(lane));
+}
+
+__global__ __launch_bounds__(256) void k_part_scatter(const int *sub, int n, int ext, int thr,
+                                                      const int *offE, const int *offH, int nblk,
+                                                      int *order) {
     __shared__ int wcnt[4];
     const int idx = blockIdx.x * 256 + threadIdx.x;
     const bool valid = idx < n;
-    const bool h = valid && sub[idx] >= thr;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(h);
-    if (lane == 0) wcnt[w] = __popcll(m);
-    __syncthreads();
-    int before = 0;
-    for (int q = 0; q < w; q++) before += wcnt[q];
-    const int hr = before + __popcll(m & lanemask_lt(lane));  // heavy rank in block
-    const int total_heavy = off[nblk];
-    const int blk_heavy = off[blockIdx.x];
+    const int v = valid ? sub[idx] : 0;
+    const bool e = valid && v >= ext;
+    const bool h = valid && v >= thr && v < ext;
+    int ne, nh;
+    const int re = block_rank(e, wcnt, &ne);
+    const int rh = block_rank(h, wcnt, &nh);
+    const int totE = offE[nblk], totH = offH[nblk];
+    const int bE = offE[blockIdx.x], bH = offH[blockIdx.x];
     if (valid) {
-        if (h)
-            order[blk_heavy + hr] = idx;
-        else
-            order[total_heavy + (blockIdx.x * 256 - blk_heavy) + (threadIdx.x - hr)] = idx;
+        if (e)
+            order[bE + re] = idx;
+        else if (h)
+            order[totE + bH + rh] = idx;
+        else  // light rank = position among the block's light bubbles
+            order[totE + totH + (blockIdx.x * 256 - bE - bH) + (threadIdx.x - re - rh)] = idx;
     }
 }
 
-void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold) {
+void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext) {
     if (b.n <= 0) return;
     const int nblk = (b.n + 255) / 256;
-    hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, b.sub, b.n, threshold, b.part);
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, b.part, nblk);
-    hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, b.sub, b.n, threshold, b.part,
-                       nblk, b.order);
+    int *cntE = b.part, *cntH = b.part + (nblk + 1);
+    hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
+                       cntE, cntH);
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntE, nblk);
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntH, nblk);
+    hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
+                       cntE, cntH, nblk, b.order);
 }
 
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,

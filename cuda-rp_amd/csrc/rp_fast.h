@@ -71,15 +71,83 @@ __device__ inline dd rsqrt_(dd x) {
     const double r = rsqrt(x.v);
     return dd(r, -0.5 * r * r * r * x.d);
 }
+#ifndef BRP_OWN_TRANSCENDENTALS
+#define BRP_OWN_TRANSCENDENTALS 1
+#endif
+
+#if BRP_OWN_TRANSCENDENTALS
+// exp(x) - 1 for -40 <= x <= -0.14 (the range alpha_coth_m1_g uses):
+// x = n ln2 + r, |r| <= ln2 / 2 (Cody-Waite reduction, ln2 in two parts),
+// e^r - 1 by its Taylor series to r^14 (truncation < 2e-19), then
+// 2^n (1 + p) - 1, which for these x is in [-1, -0.13] (no cancellation
+// beyond one ulp of 1).
+__device__ inline double expm1_neg(double x) {
+    const double n = rint(x * 1.4426950408889634);
+    double r = fma(-n, 6.93147180369123816490e-01, x);
+    r = fma(-n, 1.90821492927058770002e-10, r);
+    double p = 1.0 / 87178291200.0;  // 1/14!
+    p = fma(p, r, 1.0 / 6227020800.0);
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p * r, r, r);  // e^r - 1
+    return n == 0.0 ? p : fma(ldexp(1.0, (int)n), p, ldexp(1.0, (int)n) - 1.0);
+}
+
+// sin and cos of 0 <= x <= 64: x = k pi/2 + r, |r| <= pi/4 (Cody-Waite
+// reduction, pi/2 in two parts, each step one rounding through fma), Taylor
+// series to r^17 (sin) and r^18 (cos), truncation < 1e-19, then the
+// quadrant of k.
+__device__ inline void sincos_small(double x, double *s, double *c) {
+    const double k = rint(x * 0.63661977236758134308);
+    double r = fma(-k, 1.57079632679489655800e+00, x);
+    r = fma(-k, 6.12323399573676603587e-17, r);
+    const double r2 = r * r;
+    double ps = -1.0 / 355687428096000.0;  // -1/17!
+    ps = fma(ps, r2, 1.0 / 1307674368000.0);
+    ps = fma(ps, r2, -1.0 / 6227020800.0);
+    ps = fma(ps, r2, 1.0 / 39916800.0);
+    ps = fma(ps, r2, -1.0 / 362880.0);
+    ps = fma(ps, r2, 1.0 / 5040.0);
+    ps = fma(ps, r2, -1.0 / 120.0);
+    ps = fma(ps, r2, 1.0 / 6.0);
+    ps = fma(-(ps * r2), r, r);  // r - r^3/6 + ...
+    double pc = 1.0 / 6402373705728000.0;  // 1/18!
+    pc = fma(pc, r2, -1.0 / 20922789888000.0);
+    pc = fma(pc, r2, 1.0 / 87178291200.0);
+    pc = fma(pc, r2, -1.0 / 479001600.0);
+    pc = fma(pc, r2, 1.0 / 3628800.0);
+    pc = fma(pc, r2, -1.0 / 40320.0);
+    pc = fma(pc, r2, 1.0 / 720.0);
+    pc = fma(pc, r2, -1.0 / 24.0);
+    pc = fma(pc, r2, 0.5);
+    pc = fma(-pc, r2, 1.0);  // 1 - r^2/2 + ...
+    const int q = ((int)k) & 3;
+    const double sa = (q & 1) ? pc : ps, ca = (q & 1) ? ps : pc;
+    *s = (q & 2) ? -sa : sa;
+    *c = ((q + 1) & 2) ? -ca : ca;
+}
+__device__ inline double expm1_(double x) { return expm1_neg(x); }
+__device__ inline void sincos_(double x, double *s, double *c) { sincos_small(x, s, c); }
+#else
 __device__ inline double expm1_(double x) { return expm1(x); }
+__device__ inline void sincos_(double x, double *s, double *c) { sincos(x, s, c); }
+#endif
 __device__ inline dd expm1_(dd x) {
-    const double m = expm1(x.v);
+    const double m = expm1_(x.v);
     return dd(m, (m + 1.0) * x.d);
 }
-__device__ inline void sincos_(double x, double *s, double *c) { sincos(x, s, c); }
 __device__ inline void sincos_(dd x, dd *s, dd *c) {
     double sn, cn;
-    sincos(x.v, &sn, &cn);
+    sincos_(x.v, &sn, &cn);
     *s = dd(sn, cn * x.d);
     *c = dd(cn, -sn * x.d);
 }
@@ -89,7 +157,8 @@ struct cx {
     T re, im;
 };
 
-// alpha coth(alpha) - 1 for alpha = s (1 + i), s > 0.
+// alpha coth(alpha) - 1 for alpha = s (1 + i), s > 0.  The callers use it
+// for s > 0.1 / sqrt(2) only, so expm1_ sees -40 <= -2s < -0.14.
 template <typename T>
 __device__ inline cx<T> alpha_coth_m1_g(T s) {
     if (val(s) > 20.0) return cx<T>{s - T(1.0), s};
@@ -259,13 +328,14 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     const T PGp = fma_(c2, PGn, (c0 * Lnr) * k.PG0) * rcp_(c1);
     L.PG = PGp;
     if constexpr (std::is_same<T, double>::value) {
-        // heat and viscous dissipation (Q_B)
+        // heat and viscous dissipation (Q_B): dTdr dt with the 1/dt of its
+        // second term cancelled against the dt factor
         const double s1 = iL * k.T0_PG0R03;
-        const double dTdr =
-            Lp.re * s1 * (k.PG0R03 - 0.5 * (PGp + PGn) * Rbar * Rbar * Rbar) +
-            Lp.im * s1 * (PGp * Rp * Rp * Rp - PGn * Rn * Rn * Rn) * iom / dtL;
-        L.SumHeat -= s.four_pi_K0 * Rp * Rp * dTdr * dtL;
-        L.SumVis += s.sixteen_pi_mu * Rp * d1R * d1R * dtL;
+        const double dTdr_dt =
+            fma(Lp.re * s1 * dtL, fma(-0.5 * (PGp + PGn), Rbar * Rbar * Rbar, k.PG0R03),
+                (Lp.im * s1 * iom) * fma(PGp, Rp * Rp * Rp, -(PGn * (Rn * Rn * Rn))));
+        L.SumHeat = fma(-s.four_pi_K0 * (Rp * Rp), dTdr_dt, L.SumHeat);
+        L.SumVis = fma(s.sixteen_pi_mu * Rp * dtL, d1R * d1R, L.SumVis);
     }
     L.counter++;
 }

@@ -161,7 +161,9 @@ enum {
     BRP_BUB_PL_N = 8, BRP_BUB_PL_M = 9, BRP_BUB_Q_B = 10, BRP_BUB_N_B = 11,
     BRP_BUB_DT = 12, BRP_BUB_RE = 13, BRP_BUB_VX = 14, BRP_BUB_VY = 15,
     BRP_BUB_IBM_X = 16, BRP_BUB_IBM_Y = 17, BRP_BUB_IBN_X = 18,
-    BRP_BUB_IBN_Y = 19, BRP_BUB_R0 = 20, BRP_BUB_COUNT = 21
+    BRP_BUB_IBN_Y = 19, BRP_BUB_R0 = 20,
+    BRP_BUB_SUB = 21,   /* RP sub-steps of the bubble's last RP call (diagnostic) */
+    BRP_BUB_COUNT = 22
 };
 
 typedef struct brp_handle brp_handle;

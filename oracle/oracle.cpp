@@ -351,6 +351,7 @@ struct oracle {
     std::vector<double> bx, by, R_t, R_p, R_pn, R_n, R_nn, d1_R_p, d1_R_n, PG_p, PG_n, PL_p,
         PL_n, PL_m, Q_B, n_B, bdt, bdt_n, re, re_n, vBx, vBy, vLx, vLy;
     std::vector<int> ibmx, ibmy, ibnx, ibny;
+    std::vector<int> sub;  // RP sub-steps of the last RP call (diagnostic)
     std::vector<BubParams> bp;
 
     // frozen-schedule record / replay (mode 1 / 2): Picard counts per step and
@@ -531,7 +532,7 @@ void resize_bubbles(oracle *o, int n) {
                     &o->n_B, &o->bdt, &o->bdt_n, &o->re, &o->re_n, &o->vBx, &o->vBy, &o->vLx,
                     &o->vLy})
         v->assign(n, 0.0);
-    for (auto *v : {&o->ibmx, &o->ibmy, &o->ibnx, &o->ibny}) v->assign(n, 0);
+    for (auto *v : {&o->ibmx, &o->ibmy, &o->ibnx, &o->ibny, &o->sub}) v->assign(n, 0);
     o->bp.assign(n, BubParams{});
 }
 
@@ -917,6 +918,7 @@ int solve_bubble_radii(oracle *o, double *mean_sub, bool *capped) {
         o->bdt[b] = out.dt;
         o->re[b] = out.re;
         o->Q_B[b] = out.Q_B;
+        o->sub[b] = cnt;
         maxc = std::max(maxc, cnt);
         sum += cnt;
         cap_any = cap_any || cp;
@@ -1365,6 +1367,7 @@ int oracle_get_bubbles(const oracle *o, int32_t which, double *dst) {
         case BRP_BUB_IBM_Y: vi = &o->ibmy; break;
         case BRP_BUB_IBN_X: vi = &o->ibnx; break;
         case BRP_BUB_IBN_Y: vi = &o->ibny; break;
+        case BRP_BUB_SUB: vi = &o->sub; break;
         case BRP_BUB_R0:
             for (int b = 0; b < o->nb; b++) dst[b] = o->bp[b].R0;
             return BRP_OK;
