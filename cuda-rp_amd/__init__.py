@@ -179,14 +179,15 @@ class Simulation(_Backend):
     def restore_state(self, slot):
         self._check(self.lib.brp_restore_state(self.h, slot))
 
-    def adjoint(self, box, n_steps, window, checkpoint_every=0, n_elem=None):
+    def adjoint(self, box, n_steps, window, checkpoint_every=0, n_elem=None, tape=True):
         """brp_adjoint from the current state: J = mean p0^2 over the cell box
         (i0, i1, j0, j1) and the last `window` of `n_steps` steps, with dJ/d
         element amplitude, dJ/d element phase and dJ/d initial void-fraction
         map.  The handle ends at the state after n_steps."""
         if n_elem is None:
             n_elem = getattr(self, "_n_elem", 0)
-        ob = abi.Objective(box[0], box[1], box[2], box[3], n_steps, window, checkpoint_every, 0)
+        ob = abi.Objective(box[0], box[1], box[2], box[3], n_steps, window, checkpoint_every,
+                           0 if tape else abi.ADJ_NO_TAPE)
         ga = np.zeros(max(n_elem, 1))
         gp = np.zeros(max(n_elem, 1))
         nx, ny = self.shape("fg")[:2]

@@ -93,7 +93,10 @@ class Info(C.Structure):
 class Objective(C.Structure):
     """brp_objective: mean p0^2 over a cell box and the last `window` steps."""
     _fields_ = [(n, C.c_int32) for n in ("box_i0", "box_i1", "box_j0", "box_j1", "n_steps",
-                                          "window", "checkpoint_every", "pad_")]
+                                          "window", "checkpoint_every", "flags")]
+
+
+ADJ_NO_TAPE = 1
 
 
 class AdjointStats(C.Structure):

@@ -37,36 +37,69 @@ struct Snapshot {
     double t, tx;
 };
 
-// Intermediates of one replayed step, kept for its adjoint: the start-of-step
-// pressure and positions, every pressure evaluation's p0, every void-fraction
-// iterate (k = 0 is the prediction), and each Picard iteration's gathered
-// pressure and clamped radius.
+// Device memory carved sequentially from large chunks (the adjoint's bubble
+// tape holds thousands of small per-step arrays).
+struct Arena {
+    std::vector<void *> chunks;
+    size_t chunk_bytes = size_t(1) << 29, used = 0;
+    ~Arena() {
+        for (void *c : chunks) (void)hipFree(c);
+    }
+    size_t bytes() const { return chunks.size() * chunk_bytes; }
+    void *get(size_t bytes) {
+        bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        if (bytes > chunk_bytes) return nullptr;
+        if (chunks.empty() || used + bytes > chunk_bytes) {
+            void *c = nullptr;
+            if (hipMalloc(&c, chunk_bytes) != hipSuccess) return nullptr;
+            chunks.push_back(c);
+            used = 0;
+        }
+        void *q = (char *)chunks.back() + used;
+        used += bytes;
+        return q;
+    }
+};
+
+// Intermediates of one step, kept for its adjoint.  Grid part: the
+// start-of-step pressure and f_g, the velocities after the update, every
+// pressure evaluation's p0 and every void-fraction iterate (k = 0 is the
+// prediction).  Bubble part: the RP start state and pressure history, the
+// positions before and after the motion, and each Picard iteration's
+// gathered pressure, clamped radius and sub-step count.
 struct StepRecord {
+    bool want_grid = true, want_bub = true;
+    Arena *arena = nullptr;  // bubble part from here when set
+    // grid part
     void *p0_start = nullptr;
-    double *x_start = nullptr, *y_start = nullptr;
-    int64_t *fgC = nullptr;                 // f_g at the start (the step's f_gn)
-    double *rp[6] = {};                     // RP start state R, Rn, d1R, PG, dt, re
-    double *PLm = nullptr, *PLn = nullptr;  // liquid-pressure history of the step
+    int64_t *fgC = nullptr;                      // f_g at the start (the step's f_gn)
     void *vx_end = nullptr, *vy_end = nullptr;   // velocities after the update
-    double *x_end = nullptr, *y_end = nullptr;   // positions after the motion
     std::vector<void *> p0_it;       // [k] after evaluation k (k = 0..n_pic)
     std::vector<int64_t *> fg_it;    // [k] void-fraction iterate k
+    // bubble part
+    double *x_start = nullptr, *y_start = nullptr;
+    double *rp[6] = {};                     // RP start state R, Rn, d1R, PG, dt, re
+    double *PLm = nullptr, *PLn = nullptr;  // liquid-pressure history of the step
+    double *x_end = nullptr, *y_end = nullptr;   // positions after the motion
     std::vector<double *> PLp_it;    // [k] gathered PL_p of iteration k >= 1
     std::vector<double *> Rt_it;     // [k] R_t of iteration k >= 1
     std::vector<int *> sub_it;       // [k] RP sub-steps of iteration k >= 1, per bubble
     std::vector<void *> owned;
+    bool started = false;
     // scalars of the recorded step
-    int n_pic = 0, first = 0, cur = 0, pl_m = 0, pl_n = 0, f_cur = 0;
+    int n_pic = 0, first = 0;
     double t = 0.0;
 };
 
 // Optional control of run_steps: a forced Picard count per step (the frozen
-// schedule of a replay), a record of one step's intermediates, and the
-// Picard counts taken.
+// schedule of a replay), a record of one step's intermediates, the Picard
+// counts taken, and a bubble tape to replay instead of integrating RP (the
+// void-fraction iterates are then scattered from the taped radii).
 struct StepCtl {
     const int *force_npic = nullptr;
     StepRecord *rec = nullptr;
     std::vector<int> *npic_out = nullptr;
+    const StepRecord *tape_in = nullptr;
 };
 
 }  // namespace
@@ -508,42 +541,54 @@ struct AdjMem {
 
 int rec_grow(brp_handle *h, StepRecord *r, int k) {
     // k evaluations recorded so far need p0_it/fg_it[0..k-1] and PLp_it/Rt_it[1..k-1]
-    const size_t fb = h->fsz * h->frame;
-    auto get = [&](size_t bytes, void **out) -> int {
+    const size_t fb = h->fsz * h->frame, gb = sizeof(int64_t) * h->frame;
+    const size_t nbb = sizeof(double) * std::max(h->nb, 1);
+    auto get = [&](size_t bytes, bool bub, void **out) -> int {
         void *q = nullptr;
-        if (hipMalloc(&q, std::max<size_t>(bytes, 256)) != hipSuccess)
-            return set_err(h, BRP_ERR_DEVICE, "step record allocation failed");
-        r->owned.push_back(q);
+        if (bub && r->arena) {
+            q = r->arena->get(bytes);
+            if (!q) return set_err(h, BRP_ERR_DEVICE, "bubble tape allocation failed");
+        } else {
+            if (hipMalloc(&q, std::max<size_t>(bytes, 256)) != hipSuccess)
+                return set_err(h, BRP_ERR_DEVICE, "step record allocation failed");
+            r->owned.push_back(q);
+        }
         *out = q;
         return BRP_OK;
     };
     int rc;
-    if (!r->p0_start) {
-        const size_t nbb = sizeof(double) * std::max(h->nb, 1);
-        void *q[16];
-        const size_t sz[16] = {fb,  nbb, nbb, sizeof(int64_t) * h->frame,
-                               nbb, nbb, nbb, nbb, nbb, nbb, nbb, nbb, fb, fb, nbb, nbb};
-        for (int k = 0; k < 16; k++)
-            if ((rc = get(sz[k], &q[k]))) return rc;
-        r->p0_start = q[0];
-        r->x_start = (double *)q[1];
-        r->y_start = (double *)q[2];
-        r->fgC = (int64_t *)q[3];
-        for (int k = 0; k < 6; k++) r->rp[k] = (double *)q[4 + k];
-        r->PLm = (double *)q[10];
-        r->PLn = (double *)q[11];
-        r->vx_end = q[12];
-        r->vy_end = q[13];
-        r->x_end = (double *)q[14];
-        r->y_end = (double *)q[15];
+    if (!r->started) {
+        if (r->want_grid) {
+            void *q[4];
+            const size_t sz[4] = {fb, gb, fb, fb};
+            for (int j = 0; j < 4; j++)
+                if ((rc = get(sz[j], false, &q[j]))) return rc;
+            r->p0_start = q[0];
+            r->fgC = (int64_t *)q[1];
+            r->vx_end = q[2];
+            r->vy_end = q[3];
+        }
+        if (r->want_bub) {
+            void *q[12];
+            for (int j = 0; j < 12; j++)
+                if ((rc = get(nbb, true, &q[j]))) return rc;
+            r->x_start = (double *)q[0];
+            r->y_start = (double *)q[1];
+            for (int j = 0; j < 6; j++) r->rp[j] = (double *)q[2 + j];
+            r->PLm = (double *)q[8];
+            r->PLn = (double *)q[9];
+            r->x_end = (double *)q[10];
+            r->y_end = (double *)q[11];
+        }
+        r->started = true;
     }
-    while ((int)r->p0_it.size() < k) {
-        void *a, *b, *c = nullptr, *d = nullptr, *e = nullptr;
-        if ((rc = get(fb, &a)) || (rc = get(sizeof(int64_t) * h->frame, &b))) return rc;
-        if (!r->p0_it.empty() &&
-            ((rc = get(sizeof(double) * std::max(h->nb, 1), &c)) ||
-             (rc = get(sizeof(double) * std::max(h->nb, 1), &d)) ||
-             (rc = get(sizeof(int) * std::max(h->nb, 1), &e))))
+    while ((int)std::max(r->p0_it.size(), r->PLp_it.size()) < k) {
+        void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
+        const bool first = std::max(r->p0_it.size(), r->PLp_it.size()) == 0;
+        if (r->want_grid && ((rc = get(fb, false, &a)) || (rc = get(gb, false, &b)))) return rc;
+        if (r->want_bub && !first &&
+            ((rc = get(nbb, true, &c)) || (rc = get(nbb, true, &d)) ||
+             (rc = get(sizeof(int) * std::max(h->nb, 1), true, &e))))
             return rc;
         r->p0_it.push_back(a);
         r->fg_it.push_back((int64_t *)b);
@@ -556,7 +601,12 @@ int rec_grow(brp_handle *h, StepRecord *r, int k) {
 
 void rec_free(StepRecord &r) {
     for (void *q : r.owned) (void)hipFree(q);
+    const bool g = r.want_grid, b = r.want_bub;
+    Arena *ar = r.arena;
     r = StepRecord{};
+    r.want_grid = g;
+    r.want_bub = b;
+    r.arena = ar;
 }
 
 // ---- the coupled step ----------------------------------------------------
@@ -597,9 +647,11 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         if (rec) {
             int rc = rec_grow(h, rec, 1);
             if (rc) return rc;
-            HIPCHK(h, hipMemcpyAsync(rec->p0_start, h->p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
-            HIPCHK(h, hipMemcpyAsync(rec->fgC, h->fgb[old_cur], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
-            if (h->nb > 0) {
+            if (rec->want_grid) {
+                HIPCHK(h, hipMemcpyAsync(rec->p0_start, h->p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+                HIPCHK(h, hipMemcpyAsync(rec->fgC, h->fgb[old_cur], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
+            }
+            if (rec->want_bub && h->nb > 0) {
                 const size_t nbb = sizeof(double) * h->nb;
                 const int c = h->rp_cur;
                 double *src[10] = {h->b.x, h->b.y, h->b.R[c], h->b.Rn[c], h->b.d1R[c], h->b.PG[c],
@@ -610,10 +662,6 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                     HIPCHK(h, hipMemcpyAsync(dst[q], src[q], nbb, hipMemcpyDeviceToDevice, s));
             }
             rec->first = h->first_step;
-            rec->cur = h->rp_cur;
-            rec->pl_m = h->pl_m;
-            rec->pl_n = h->pl_n;
-            rec->f_cur = old_cur;
             rec->t = h->t;
         }
         HIPCHK(h, hipEventRecord(h->ev[0], s));
@@ -629,7 +677,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                  h->rxp_d, h->sc_d);
         launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
         HIPCHK(h, hipEventRecord(h->ev[3], s));
-        if (rec) {
+        if (rec && rec->want_grid) {
             HIPCHK(h, hipMemcpyAsync(rec->p0_it[0], p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
             HIPCHK(h, hipMemcpyAsync(rec->fg_it[0], h->fgb[old_m], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
         }
@@ -659,13 +707,18 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                     if (rc) return rc;
                 }
                 HIPCHK(h, hipEventRecord(h->ev[7], s));
-                launch_order_partition(s, h->b, h->bp.heavy_threshold, h->bp.extreme_threshold);
+                const StepRecord *tin = ctl ? ctl->tape_in : nullptr;
+                if (!tin) launch_order_partition(s, h->b, h->bp.heavy_threshold, h->bp.extreme_threshold);
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
-                launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
-                                      p0, h->fgb[target], h->sc_d, loop);
+                if (tin)  // replay: the void fraction from the taped radii, no RP
+                    launch_scatter_radii(s, h->g, h->bp, h->b, tin->Rt_it[loop], h->fgb[target], h->sc_d);
+                else
+                    launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n,
+                                          h->pl_p, p0, h->fgb[target], h->sc_d, loop);
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
-                if (rec) {
+                if (rec && rec->want_grid)
                     HIPCHK(h, hipMemcpyAsync(rec->fg_it[loop], h->fgb[target], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
+                if (rec && rec->want_bub) {
                     HIPCHK(h, hipMemcpyAsync(rec->PLp_it[loop], h->b.PL[h->pl_p], sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
                     HIPCHK(h, hipMemcpyAsync(rec->Rt_it[loop], h->b.R_t, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
                     HIPCHK(h, hipMemcpyAsync(rec->sub_it[loop], h->b.sub, sizeof(int) * h->nb, hipMemcpyDeviceToDevice, s));
@@ -676,7 +729,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                           h->fgb[target], h->fgb[old_cur], h->fgb[prev], h->W, ax,
                                           bx, ay, by, h->xu_d, h->rxp_d, h->sc_d, loop);
                 launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
-                if (rec)
+                if (rec && rec->want_grid)
                     HIPCHK(h, hipMemcpyAsync(rec->p0_it[loop], p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
                 HIPCHK(h, hipEventRecord(h->ev[6], s));
                 HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr, hipMemcpyDeviceToHost, s));
@@ -731,9 +784,11 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         st.n_pic = loop;
         if (rec) {
             rec->n_pic = loop;
-            HIPCHK(h, hipMemcpyAsync(rec->vx_end, vx, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
-            HIPCHK(h, hipMemcpyAsync(rec->vy_end, vy, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
-            if (h->nb > 0) {
+            if (rec->want_grid) {
+                HIPCHK(h, hipMemcpyAsync(rec->vx_end, vx, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+                HIPCHK(h, hipMemcpyAsync(rec->vy_end, vy, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+            }
+            if (rec->want_bub && h->nb > 0) {
                 HIPCHK(h, hipMemcpyAsync(rec->x_end, h->b.x, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
                 HIPCHK(h, hipMemcpyAsync(rec->y_end, h->b.y, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
             }
@@ -780,14 +835,15 @@ struct AdjState {
     double *Lx = nullptr, *Ly = nullptr;
     double *lRt = nullptr, *lPLp = nullptr, *lPLn = nullptr, *lPLm = nullptr;
     unsigned int *mismatch = nullptr;  // RP replays whose sub-step count differed
+    int *order = nullptr, *part = nullptr;  // longest-first order of the RP adjoint
 };
 
 // Reverse of one replayed step.  On entry `a` holds the adjoint of the state
 // after the step and the handle holds that state with `rec` its
 // intermediates; on exit `a` holds the adjoint of the state before it.
 template <typename F>
-void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, const int box[4],
-                  AdjState &a) {
+void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, bool seed,
+                  double cJ, const int box[4], AdjState &a) {
     hipStream_t s = h->stream;
     const GridParams &g = h->g;
     const bool bub = h->cfg.bub_enable && h->nb > 0;
@@ -798,16 +854,16 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, co
     // the bubble arrays as the step saw them: positions after the motion, RP
     // start state in slot 0, PL history in slots 0 (m) and 1 (n)
     BubbleArrays bv = h->b;
-    bv.x = rec.x_end;
-    bv.y = rec.y_end;
-    bv.R[0] = rec.rp[0];
-    bv.Rn[0] = rec.rp[1];
-    bv.d1R[0] = rec.rp[2];
-    bv.PG[0] = rec.rp[3];
-    bv.bdt[0] = rec.rp[4];
-    bv.re[0] = rec.rp[5];
-    bv.PL[0] = rec.PLm;
-    bv.PL[1] = rec.PLn;
+    bv.x = brec.x_end;
+    bv.y = brec.y_end;
+    bv.R[0] = brec.rp[0];
+    bv.Rn[0] = brec.rp[1];
+    bv.d1R[0] = brec.rp[2];
+    bv.PG[0] = brec.rp[3];
+    bv.bdt[0] = brec.rp[4];
+    bv.re[0] = brec.rp[5];
+    bv.PL[0] = brec.PLm;
+    bv.PL[1] = brec.PLn;
     if (seed) launch_adj_seed<F>(s, g, (const F *)rec.p0_it[n], a.f.p0, box, cJ);
     // start-of-step f_g adjoints: f_gn' = f_g and f_gm' = f_gn of the start
     double *LgC = a.LfN, *LfNs = a.LfM, *LfMs = a.LfC;
@@ -816,9 +872,14 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, co
         launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
         launch_adj_pressure<F>(s, g, false, a.f, rec.fg_it[k], gC, k == n ? a.LfC : nullptr, a.Lf,
                                LgC, nullptr, nullptr, ax, bx, ay, by);
-        launch_adj_scatter(s, g, h->bp, bv, rec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
-        launch_adj_rp(s, g, h->bp, bv, 0, 0, 1, rec.PLp_it[k], a.lRt, k == n ? a.LR : none,
-                      a.LRs, a.lPLp, a.lPLn, a.lPLm, rec.sub_it[k], a.mismatch);
+        launch_adj_scatter(s, g, h->bp, bv, brec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
+        BubbleArrays bo = bv;  // work order from the forward's sub-step counts
+        bo.sub = brec.sub_it[k];
+        bo.order = a.order;
+        bo.part = a.part;
+        launch_order_partition(s, bo, h->bp.heavy_threshold, h->bp.extreme_threshold);
+        launch_adj_rp(s, g, h->bp, bv, 0, 0, 1, brec.PLp_it[k], a.lRt, k == n ? a.LR : none,
+                      a.LRs, a.lPLp, a.lPLn, a.lPLm, brec.sub_it[k], a.mismatch, a.order);
         launch_adj_gather<F>(s, g, h->bp, bv, (const F *)rec.p0_it[k - 1], a.lPLp,
                              k == n ? a.LA : nullptr, a.f.p0, a.Lx, a.Ly);
     }
@@ -826,7 +887,7 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, bool seed, double cJ, co
     launch_adj_pressure<F>(s, g, true, a.f, rec.fg_it[0], gC, n == 0 ? a.LfC : nullptr, nullptr,
                            LgC, LfNs, LfMs, ax, bx, ay, by);
     if (bub)
-        launch_adj_motion<F>(s, g, h->bp, h->nb, rec.x_start, rec.y_start, (const F *)rec.vx_end,
+        launch_adj_motion<F>(s, g, h->bp, h->nb, brec.x_start, brec.y_start, (const F *)rec.vx_end,
                              (const F *)rec.vy_end, a.f.vx, a.f.vy, a.Lx, a.Ly);
     launch_adj_velocity_ghosts(s, g, a.f.vx, a.f.vy);
     launch_adj_velocity<F>(s, g, a.f.vx, a.f.vy, (const F *)rec.p0_start, gC, rec.first,
@@ -864,17 +925,37 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
     AdjMem mem;
     std::vector<Snapshot> ck(nseg);
     Snapshot fin;
-    std::vector<StepRecord> recs(K);
+    std::vector<StepRecord> recs(K);  // grid part (and, without the tape, bubble part)
+    // Bubble tape: the bubble part of every step, recorded by the forward
+    // sweep, so that recomputing a segment replays the grid only (the RP
+    // integrations, the dominant cost, are not repeated).  Used while it
+    // fits in tape_budget bytes; otherwise each segment's recomputation
+    // integrates RP again and records the bubble part itself.
+    const bool bub = h->cfg.bub_enable && h->nb > 0;
+    Arena arena;
+    std::vector<StepRecord> tape;
+    const size_t tape_budget = size_t(96) << 30;
+    bool use_tape = bub && !(ob.flags & BRP_ADJ_NO_TAPE) &&
+                    (size_t)n * (size_t)h->nb * 8 * 24 < tape_budget;
+    if (use_tape) {
+        tape.resize(n);
+        for (auto &t : tape) {
+            t.want_grid = false;
+            t.arena = &arena;
+        }
+        for (auto &r : recs) r.want_bub = false;
+    }
     struct Cleanup {
         std::vector<Snapshot> *a;
         Snapshot *c;
-        std::vector<StepRecord> *r;
+        std::vector<StepRecord> *r, *t;
         ~Cleanup() {
             for (auto &x : *a) snap_free(x);
             snap_free(*c);
             for (auto &x : *r) rec_free(x);
+            for (auto &x : *t) rec_free(x);
         }
-    } cleanup{&ck, &fin, &recs};
+    } cleanup{&ck, &fin, &recs, &tape};
     int rc;
     double *jacc = nullptr;
     if ((rc = mem.get(h, &jacc, (size_t)n))) return rc;
@@ -889,6 +970,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         if ((rc = snap_save(h, ck[sg]))) return rc;
         const int s_end = std::min(n, (sg + 1) * K);
         for (int st = sg * K + 1; st <= s_end; st++) {
+            fctl.rec = use_tape ? &tape[st - 1] : nullptr;
             if ((rc = run_steps<F>(h, 1, nullptr, &fctl))) return rc;
             if (st > n - ob.window) launch_box_sumsq<F>(s, h->g, (const F *)h->p0, box, jacc + st - 1);
         }
@@ -914,7 +996,9 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
                        &a.LRs[3], &a.LA, &a.LB, &a.Lx, &a.Ly, &a.lRt, &a.lPLp, &a.lPLn, &a.lPLm};
     for (double **q : bubs)
         if ((rc = mem.get(h, q, nb))) return rc;
-    if ((rc = mem.get(h, &a.mismatch, 1))) return rc;
+    if ((rc = mem.get(h, &a.mismatch, 1)) || (rc = mem.get(h, &a.order, nb)) ||
+        (rc = mem.get(h, &a.part, 2 * (nb / 256 + 2))))
+        return rc;
 
     // reverse sweep: per segment, the steps are recomputed from the segment's
     // checkpoint with their intermediates recorded, then reversed
@@ -925,13 +1009,15 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
             StepCtl c;
             c.force_npic = &npic[st - 1];
             c.rec = &recs[st - s0];
+            c.tape_in = use_tape ? &tape[st - 1] : nullptr;
             if ((rc = run_steps<F>(h, 1, nullptr, &c))) return rc;
             if (recs[st - s0].n_pic != npic[st - 1])
                 return set_err(h, BRP_ERR_STATE, "replay of step %d took %d Picard iterations, not %d",
                                st, recs[st - s0].n_pic, npic[st - 1]);
         }
         for (int st = s1; st >= s0; st--)
-            adjoint_step<F>(h, recs[st - s0], st > n - ob.window, cJ, box, a);
+            adjoint_step<F>(h, recs[st - s0], use_tape ? tape[st - 1] : recs[st - s0],
+                            st > n - ob.window, cJ, box, a);
     }
     HIPCHK(h, hipStreamSynchronize(s));
     const auto t2 = clk::now();
@@ -971,11 +1057,11 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         stats->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
         // step records: per evaluation p0 and f_g, per iteration PL_p, R_t and
         // sub-step counts, per step start/end fields and bubble state
-        size_t recb = 0;
+        size_t recb = arena.bytes();
         for (const StepRecord &r : recs)
-            if (r.p0_start)
-                recb += r.p0_it.size() * ((h->fsz + 8) * fr + 20 * nb) +
-                        (3 * h->fsz + 8) * fr + 14 * 8 * nb;
+            if (r.started)
+                recb += r.p0_it.size() * (h->fsz + 8) * fr + (3 * h->fsz + 8) * fr +
+                        (r.want_bub ? r.p0_it.size() * 20 * nb + 12 * 8 * nb : 0);
         stats->memory_bytes = (int64_t)((nseg + 1) * snap_bytes(h) + mem.bytes + recb);
         stats->n_checkpoints = nseg;
         stats->segment = K;

@@ -131,6 +131,8 @@ void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold,
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);
+void launch_scatter_radii(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                          const BubbleArrays &b, const double *Rt, int64_t *fg, StepScalars *sc);
 void launch_copy_d(hipStream_t s, double *dst, const double *src, int n);
 void launch_bubble_index(hipStream_t s, const GridParams &g, const BubbleArrays &b,
                          double *out, int which);
@@ -174,7 +176,8 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
 void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
                    const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
-                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch);
+                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch,
+                   const int *order);
 void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,
                            double *lPLn, double *lPLm);
 void launch_fill_d(hipStream_t s, double *p, size_t n, double v);

@@ -479,7 +479,9 @@ __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams b
 // LRe (the adjoint of the step's final RP state) is null except for the last
 // Picard iteration.  Outputs: LRs[d] += (d < 4), lPLp = (d = 4),
 // lPLn += (d = 5), lPLm += (d = 6); sub (if not null) holds the forward's
-// sub-step counts, and a lane-7 mismatch is counted in *mismatch.
+// sub-step counts, and a lane-7 mismatch is counted in *mismatch.  order
+// (if not null) lists the bubbles longest-first, so the long integrations
+// start in the first workgroups.
 __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, BubbleArrays b,
                                                  int cur, int plm, int pln, const double *PLp,
                                                  const double *lRt, const double *LRe0,
@@ -487,10 +489,10 @@ __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, B
                                                  const double *LRe3, double *LRs0, double *LRs1,
                                                  double *LRs2, double *LRs3, double *lPLp,
                                                  double *lPLn, double *lPLm, const int *sub,
-                                                 unsigned int *mismatch) {
+                                                 unsigned int *mismatch, const int *order) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int idx = gid >> 3, d = gid & 7;
-    if (idx >= b.n) return;
+    if ((gid >> 3) >= b.n) return;
+    const int idx = order ? order[gid >> 3] : gid >> 3, d = gid & 7;
     const RPFastShared fs{bp.PL0,
                           bp.gam,
                           bp.mu,
@@ -646,12 +648,13 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
 void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
                    const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
-                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch) {
+                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch,
+                   const int *order) {
     if (b.n <= 0) return;
     const long threads = (long)b.n * 8;
     hipLaunchKernelGGL(k_adj_rp, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g, bp,
                        b, cur, plm, pln, PLp, lRt, LRe[0], LRe[1], LRe[2], LRe[3], LRs[0], LRs[1],
-                       LRs[2], LRs[3], lPLp, lPLn, lPLm, sub, mismatch);
+                       LRs[2], LRs[3], lPLp, lPLn, lPLm, sub, mismatch, order);
 }
 
 void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,

@@ -431,6 +431,22 @@ __global__ __launch_bounds__(256) void k_scatter_initial(GridParams g, BubblePar
     if (bad) atomicOr(&sc->flags, 2);
 }
 
+// The scatter of given clamped radii (the persistent kernel's scatter without
+// its RP integration): every bubble's B x B stencil at its current position.
+__global__ __launch_bounds__(256) void k_scatter_radii(GridParams g, BubbleParams bp,
+                                                        BubbleArrays b, const double *Rt,
+                                                        int64_t *fg, StepScalars *sc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (idx < b.n) {
+        const double x = b.x[idx], y = b.y[idx];
+        const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+        scatter_bubble(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], Rt[idx], x), fg,
+                       &bad);
+    }
+    if (bad) atomicOr(&sc->flags, 2);
+}
+
 // VoidFractionCylinderKernel (bubble_CUDA_kernel.cuh:559-573) for order 2:
 // the axis fold f_g(0, j) += f_g(1, j); f_g(1, j) = f_g(0, j), exact in int64.
 __global__ void k_cyl_fold(GridParams g, int64_t *fg) {
@@ -604,6 +620,13 @@ void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubblePara
     if (b.n <= 0) return;
     hipLaunchKernelGGL(k_scatter_initial, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b,
                        fg_target, sc);
+}
+
+void launch_scatter_radii(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                          const BubbleArrays &b, const double *Rt, int64_t *fg, StepScalars *sc) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_radii, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, Rt, fg,
+                       sc);
 }
 
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg) {

@@ -253,11 +253,17 @@ int brp_get_phase_times(const brp_handle *h, double ms[5]);
  * over the final p0 of each of the last `window` of `n_steps` steps.  The
  * forward run keeps a checkpoint every `checkpoint_every` steps
  * (0: ceil(sqrt(n_steps))) and recomputes each segment during the reverse
- * sweep. */
+ * sweep; the bubbles' RP results of every step are kept on a device tape
+ * (when it fits in 96 GB and flags lacks BRP_ADJ_NO_TAPE), so that the
+ * recomputation replays the grid only. */
 typedef struct brp_objective {
     int32_t box_i0, box_i1, box_j0, box_j1;
-    int32_t n_steps, window, checkpoint_every, pad_;
+    int32_t n_steps, window, checkpoint_every;
+    int32_t flags;  /* BRP_ADJ_NO_TAPE: recompute RP in the reverse sweep
+                       instead of keeping a tape of every step's bubble
+                       intermediates (less memory, about one forward more) */
 } brp_objective;
+#define BRP_ADJ_NO_TAPE 1
 
 typedef struct brp_adjoint_stats {
     double J;

@@ -82,15 +82,18 @@ def test_adjoint_matches_frozen_schedule_differences(brp):
 
 
 def test_adjoint_checkpointing_and_final_state(brp):
-    """The gradient does not depend on the checkpoint spacing, and the handle
-    ends at exactly the state a plain forward run reaches."""
+    """The gradient does not depend on the checkpoint spacing nor on whether
+    the reverse sweep replays a bubble tape or integrates RP again, and the
+    handle ends at exactly the state a plain forward run reaches."""
     cfg, bub, src = cases.adjoint_case(nb=24)
     g1, r1 = _gpu_adjoint(brp, cfg, bub, src, every=0)   # sqrt spacing
     g2, r2 = _gpu_adjoint(brp, cfg, bub, src, every=STEPS)  # one segment
     g3, r3 = _gpu_adjoint(brp, cfg, bub, src, every=7)
+    g4 = cases.make(brp.Simulation, cfg, bub, src)   # RP recomputed instead of taped
+    r4 = g4.adjoint(cases.focal_box(cfg), STEPS, WINDOW, 20, tape=False)
     f = cases.make(brp.Simulation, cfg, bub, src)
     f.step(STEPS)
-    for r in (r2, r3):
+    for r in (r2, r3, r4):
         assert r["J"] == r1["J"]
         for k in ("grad_amp", "grad_phase", "grad_fg0"):
             np.testing.assert_allclose(r[k], r1[k], rtol=1e-11,
