@@ -148,11 +148,12 @@ def run_gpu(brp, cfg, bub, warmup, steps, first):
     t_first = time.perf_counter() - t0
     if warmup > first:
         stepped(sim, warmup - first, "warmup", chunk=50)
+    at_warmup = (sim.field("p0"), sim.bubbles("R_t"))
     log(f"  warmup done ({warmup} steps), timing {steps} steps")
     t0 = time.perf_counter()
     st, phases = stepped(sim, steps, "timed")
     wall = time.perf_counter() - t0
-    return sim, st, wall, phases, t_first, st_first
+    return sim, st, wall, phases, t_first, at_warmup
 
 
 def cpu_baseline(abi, cfg, bub, sample_steps, threads):
@@ -165,6 +166,34 @@ def cpu_baseline(abi, cfg, bub, sample_steps, threads):
     wall = time.perf_counter() - t0
     o.close()
     return wall, st
+
+
+def measured_traffic():
+    """HBM bytes per RP launch (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
+    from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "k_bubble_persistent_2048_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def grid_roofline(ph, n, steps, mixed):
+    """The grid kernels (velocity, first pressure evaluation and their ghost
+    kernels) against HBM: 20 + 32 algorithmic bytes per cell-update in mixed
+    mode (vx, vy, p0 read, vx, vy written; vx, vy, pn.x, pn.y, p0 read, p.x,
+    p.y, p0 written), twice that in FP64; the Picard re-evaluations cover the
+    bubble window only and are left out of the byte count."""
+    per_cell = 52.0 if mixed else 104.0
+    ms = (ph["velocity"] + ph["pressure"]) / steps
+    achieved = per_cell * n * n / (ms * 1e-3) / 1e9
+    return {"kernels": "k_velocity + k_pressure_first + ghosts (+ window Picard evaluations)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "ms_per_step": ms,
+            "bytes_per_cell_update": per_cell}
 
 
 def calibrate_copy(mib=1024, reps=5):
@@ -222,8 +251,8 @@ def main():
 
     cfg_m, bub = c2_workload(abi, abi.PREC_MIXED, n, nb)
     log(f"mixed: {n}^2 cells, {nb} bubbles, warmup {args.warmup}, steps {args.steps}")
-    sim_m, st_m, wall_m, ph_m, tfirst_m, _ = run_gpu(brp, cfg_m, bub, args.warmup, args.steps,
-                                                     first)
+    sim_m, st_m, wall_m, ph_m, tfirst_m, warm_m = run_gpu(brp, cfg_m, bub, args.warmup,
+                                                          args.steps, first)
     cells = n * n
     cups = cells * args.steps / wall_m
     bsps = nb * args.steps / wall_m
@@ -258,19 +287,24 @@ def main():
         "picard_per_step": launches / args.steps,
         "rp_substeps_per_s": substeps / wall_m,
         "phase_ms_per_step": {k: v / args.steps for k, v in ph_m.items()},
-        "roofline": {"kernel": "k_bubble_step (gather + Rayleigh-Plesset + scatter)",
+        "roofline": {"kernel": "k_bubble_persistent (gather + Rayleigh-Plesset + scatter)",
                      "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "traffic": measured_traffic(), "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch,
-                     "note": "FP64 vector ALU bound; MI355X FP64 vector and matrix peaks "
-                             "are both 78.6 TFLOP/s"},
+                     "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both "
+                             "78.6 TFLOP/s.  A launch is bound by the longest bubbles' "
+                             "sequential sub-step chains (latency), not by throughput; "
+                             "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / "
+                             "WRITE_SIZE (profiles/)"},
+        "grid_roofline": grid_roofline(ph_m, n, args.steps, mixed=True),
     }
 
     if not args.no_fp64:
         cfg_d, _ = c2_workload(abi, abi.PREC_FP64, n, nb)
         log("fp64 run")
-        sim_d, st_d, wall_d, ph_d, _, _ = run_gpu(brp, cfg_d, bub, args.warmup, args.steps, 0)
+        sim_d, st_d, wall_d, ph_d, _, warm_d = run_gpu(brp, cfg_d, bub, args.warmup, args.steps,
+                                                       0)
         cups_d = cells * args.steps / wall_d
         from cases import rel_l2
         err_p0 = rel_l2(sim_m.field("p0"), sim_d.field("p0"))
@@ -278,9 +312,14 @@ def main():
         out["fp64"] = {"value": cups_d, "ms_per_step": wall_d / args.steps * 1e3,
                        "bubble_steps_per_s": nb * args.steps / wall_d,
                        "phase_ms_per_step": {k: v / args.steps for k, v in ph_d.items()},
-                       "picard_per_step": sum(s["n_pic"] for s in st_d) / args.steps}
+                       "picard_per_step": sum(s["n_pic"] for s in st_d) / args.steps,
+                       "grid_roofline": grid_roofline(ph_d, n, args.steps, mixed=False)}
         out["mixed_vs_fp64"] = {"speedup": cups / cups_d, "p0_rel_l2": err_p0,
                                 "R_t_rel_l2": err_rt,
+                                "at_step": args.warmup + args.steps,
+                                "p0_rel_l2_at_warmup_end": rel_l2(warm_m[0], warm_d[0]),
+                                "R_t_rel_l2_at_warmup_end": rel_l2(warm_m[1], warm_d[1]),
+                                "warmup_end_step": args.warmup,
                                 "picard_count_mismatch_steps":
                                     int(sum(a["n_pic"] != b["n_pic"] for a, b in zip(st_m, st_d)))}
         log(f"  fp64: {wall_d / args.steps * 1e3:.3f} ms/step; p0 err {err_p0:.2e}")
