@@ -127,6 +127,7 @@ struct brp_handle {
     void *ax = nullptr, *bx = nullptr, *ay = nullptr, *by = nullptr;
     double *xu_d = nullptr, *rxp_d = nullptr;
     double *el_amp_d = nullptr, *el_phase_d = nullptr;
+    double *T_d = nullptr, *dT_d = nullptr, *heat_d = nullptr;  // temperature solve
     int n_elem = 0;
 
     // bubbles
@@ -753,6 +754,16 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 }
             }
         }
+        // calculate_temperature (bubble_CUDA.cu:279-282): the final f_g and p,
+        // the previous step's properties
+        if (h->cfg.thermal) {
+            HIPCHK(h, hipEventRecord(h->ev[7], s));
+            launch_temperature<F>(s, h->g, h->bp, h->b, bub, h->W, h->T_inf, h->T_d, h->dT_d,
+                                  h->heat_d, h->fgb[prev], h->fgb[old_cur], h->first_step,
+                                  (const F *)h->pb[p_i][0], (const F *)h->pb[p_i][1],
+                                  (const F *)h->pb[pn_i][0], (const F *)h->pb[pn_i][1]);
+            HIPCHK(h, hipEventRecord(h->ev[4], s));
+        }
         // rotate the void-fraction buffers: final iterate in `prev`, zero buffer in `target`
         h->fcur = prev;
         h->fz = target;
@@ -773,6 +784,11 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
             h->phase_ms[1] += b2;
             h->phase_ms[2] += c + pic_p_ms;
             h->phase_ms[3] += bub_ms;
+            if (h->cfg.thermal) {
+                float th = 0.f;
+                (void)hipEventElapsedTime(&th, h->ev[7], h->ev[4]);
+                other_ms += th;
+            }
             h->phase_ms[4] += other_ms;
         }
         const int flags = h->sc_h->flags;
@@ -1107,7 +1123,7 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     if (!(c.sim_cfl > 0)) return set_err(nullptr, BRP_ERR_INVALID, "simulation.cfl must be > 0");
     if (c.precision != BRP_PREC_FP64 && c.precision != BRP_PREC_MIXED)
         return set_err(nullptr, BRP_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
-    if (c.thermal) return set_err(nullptr, BRP_ERR_INVALID, "thermal solve is not available in this build");
+
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return set_err(nullptr, BRP_ERR_NO_DEVICE, "no HIP device visible");
@@ -1216,6 +1232,11 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
 
     int rc = h->prec == BRP_PREC_FP64 ? alloc_grid<double>(h) : alloc_grid<float>(h);
     if (rc) return fail(rc);
+    if (c.thermal &&
+        ((rc = dalloc(h, (void **)&h->T_d, sizeof(double) * h->frame)) ||
+         (rc = dalloc(h, (void **)&h->dT_d, sizeof(double) * h->frame)) ||
+         (rc = dalloc(h, (void **)&h->heat_d, sizeof(double) * h->frame))))
+        return fail(rc);
     if ((rc = dalloc(h, (void **)&h->sc_d, sizeof(StepScalars)))) return fail(rc);
     if (hipHostMalloc((void **)&h->sc_h, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipHostMalloc failed"));
@@ -1397,7 +1418,14 @@ int brp_get_field(brp_handle *h, int32_t field, double *dst, int64_t ld) {
                 for (double &v : fr) v = (1.0 - v) * h->rho_inf;
             }
             break;
-        case BRP_FIELD_T: fr.assign(h->frame, 0.0); break;
+        case BRP_FIELD_T:
+            if (h->T_d) {
+                fr.resize(h->frame);
+                HIPCHK(h, hipMemcpy(fr.data(), h->T_d, sizeof(double) * h->frame, hipMemcpyDeviceToHost));
+            } else {
+                fr.assign(h->frame, 0.0);
+            }
+            break;
         default: return BRP_ERR_INVALID;
     }
     if (rc) return rc;
@@ -1431,6 +1459,15 @@ int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld) {
             HIPCHK(h, hipMemcpy(h->fgb[a], fr.data(), sizeof(int64_t) * h->frame, hipMemcpyHostToDevice));
         }
         HIPCHK(h, hipMemset(h->fgb[h->fz], 0, sizeof(int64_t) * h->frame));
+        return BRP_OK;
+    }
+    if (field == BRP_FIELD_T) {
+        if (!h->T_d) return set_err(h, BRP_ERR_INVALID, "T is not settable with thermal = 0");
+        std::vector<double> fr(h->frame);
+        HIPCHK(h, hipMemcpy(fr.data(), h->T_d, sizeof(double) * h->frame, hipMemcpyDeviceToHost));
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) fr[frame_off(i + i0, j + j0, h->pitch)] = src[(size_t)j * ld + i];
+        HIPCHK(h, hipMemcpy(h->T_d, fr.data(), sizeof(double) * h->frame, hipMemcpyHostToDevice));
         return BRP_OK;
     }
     void *dstp = nullptr;
@@ -1528,9 +1565,8 @@ int brp_adjoint(brp_handle *h, const brp_objective *obj, double *grad_amp, doubl
         o.box_j1 > h->M || o.box_j0 > o.box_j1)
         return set_err(h, BRP_ERR_INVALID, "adjoint: focal box outside the interior");
     if (grad_fg0 && ld < h->N + 2) return set_err(h, BRP_ERR_INVALID, "adjoint: ld < N + 2");
-    if (h->cfg.tr_cylindrical || h->cfg.sim_order != 2 || h->cfg.thermal)
-        return set_err(h, BRP_ERR_INVALID,
-                       "adjoint: Cartesian, simulation.order = 2, thermal off only");
+    if (h->cfg.tr_cylindrical || h->cfg.sim_order != 2)
+        return set_err(h, BRP_ERR_INVALID, "adjoint: Cartesian, simulation.order = 2 only");
     int rc = brp_prepare(h);
     if (rc) return rc;
     return h->prec == BRP_PREC_FP64

@@ -138,6 +138,13 @@ void launch_bubble_index(hipStream_t s, const GridParams &g, const BubbleArrays 
                          double *out, int which);
 void launch_zero_scalars(hipStream_t s, StepScalars *sc, int nslots);
 
+// ---- temperature (kernels_thermal.hip) ----
+template <typename F>
+void launch_temperature(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                        const BubbleArrays &b, bool bubbles, Window w, double T_inf, double *T,
+                        double *dT, double *heat, const int64_t *fg, const int64_t *fgn,
+                        int first, const F *px, const F *py, const F *pnx, const F *pny);
+
 // ---- adjoint (kernels_adjoint.hip) ----
 // Adjoint fields, FP64, same frame as the forward fields.
 struct AdjField {

@@ -140,6 +140,7 @@ const std::vector<Opt> &file_options() {
         {"extensions.max_picard", INT, I32(max_picard), " (=1000)"},
         {"extensions.max_substeps", INT, I32(max_substeps), " (=1000000)"},
         {"extensions.rp_exact", BOOL, B32(rp_exact), " (=0)"},
+        {"extensions.thermal", BOOL, B32(thermal), " (=1)"},
     };
     return v;
 }
@@ -192,6 +193,7 @@ int store_generic(Options &o, const std::string &name, const std::string &value,
 
 int parse_command_line(int argc, char **argv, Options *o, std::string *err) {
     brp_config_default(&o->cfg);
+    o->cfg.thermal = 1;  // the reference computes the temperature every step
     o->stored_names.clear();
     std::set<std::string> seen;
     struct Long {
@@ -392,7 +394,7 @@ std::string options_json(const Options &o) {
       << ", " << o.dbg_fg << ", " << o.dbg_T << ", " << o.dbg_p0 << ", " << o.dbg_vxy << ", "
       << o.dbg_bubbles << "], \"extensions\": [" << c.precision << ", " << c.ext_source_in_pml
       << ", " << c.ext_focus_x_set << ", " << c.ext_focus_x << ", " << c.max_picard << ", "
-      << c.max_substeps << ", " << c.rp_exact << "], \"ascii\": " << o.ascii
+      << c.max_substeps << ", " << c.rp_exact << ", " << c.thermal << "], \"ascii\": " << o.ascii
       << ", \"sph\": " << o.sph << ", \"directory\": \"" << o.directory
       << "\", \"target_dir\": \"" << o.target_dir << "\", \"input_file\": \"" << o.input_file
       << "\", \"ignored\": [";

@@ -105,7 +105,9 @@ typedef struct brp_config {
     int32_t precision;           /* BRP_PREC_FP64 / BRP_PREC_MIXED           */
     int32_t max_picard;          /* Picard cap per step (0 -> 1000)          */
     int32_t max_substeps;        /* RP sub-step cap per call (0 -> 1000000)  */
-    int32_t thermal;             /* run the one-way temperature solve        */
+    int32_t thermal;             /* run the one-way temperature solve
+                                    (calculate_temperature, which the reference
+                                    runs every step; 0 skips it)              */
     int32_t rp_exact;            /* 1: RP in the reference's operation order;
                                     0: same model, shorter dependency chain   */
     int32_t pad1_;

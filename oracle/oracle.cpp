@@ -948,6 +948,102 @@ void store_variables(oracle *o) {
     o->fg.v = o->Work.v;
 }
 
+// ---- temperature (calculate_temperature, bubble_CUDA_kernel.cuh:1347-1380) --
+// MixtureKMKernel (bubble_CUDA_kernel.cuh:955-969): k_m on the 2m range.
+void mixture_km(oracle *o) {
+    const ArrayIndex &a = o->a;
+    for (int j = a.jsta2m; j <= a.jend2m; j++)
+        for (int i = a.ista2m; i <= a.iend2m; i++)
+            o->k_m(i, j) = thermal_conductivity_water(o->T(i, j) + o->T_inf) *
+                           (1.0 - 3.0 * o->fg(i, j) / (2.0 + o->fg(i, j)));
+}
+
+// BubbleHeatKernel (bubble_CUDA_kernel.cuh:972-1006): Q_B of every bubble
+// spread over its B x B stencil into Work, in bubble order.
+void bubble_heat(oracle *o) {
+    const ArrayIndex &a = o->a;
+    for (int b = 0; b < o->nb; b++) {
+        double Q_temp;
+        if (o->cfg.tr_cylindrical)
+            Q_temp = o->n_B[b] * o->Q_B[b] * o->rdx * o->rdy / o->bx[b];
+        else
+            Q_temp = o->n_B[b] * o->Q_B[b] * o->rdx * o->rdy;
+        for (int i = o->ibnx[b] + o->mbs; i <= o->ibnx[b] + o->mbe; i++) {
+            const double Dx = delta_x(o, i, o->bx[b]);
+            for (int j = o->ibny[b] + o->mbs; j <= o->ibny[b] + o->mbe; j++) {
+                const double Dy = delta_y(o, j, o->by[b]);
+                if (i >= a.ista2m && i <= a.iend2m && j >= a.jsta2m && j <= a.jend2m)
+                    o->Work(i, j) += Q_temp * Dx * Dy;
+            }
+        }
+    }
+}
+
+// MixtureEnergyKernel (bubble_CUDA_kernel.cuh:1009-1033): conductive fluxes
+// with the harmonic mean of k_m on the faces.
+void mixture_energy(oracle *o) {
+    const ArrayIndex &a = o->a;
+    for (int j = a.jsta1m; j <= a.jend1m; j++)
+        for (int i = a.ista1n; i <= a.iend1n; i++) {
+            double s0 = 2.0 * o->k_m(i, j) * o->k_m(i + 1, j) / (o->k_m(i, j) + o->k_m(i + 1, j));
+            double s1 = (-o->T(i, j) + o->T(i + 1, j)) * o->rdx;
+            o->Ex(i, j) = s0 * s1;
+        }
+    for (int j = a.jsta1n; j <= a.jend1n; j++)
+        for (int i = a.ista1m; i <= a.iend1m; i++) {
+            double s0 = 2.0 * o->k_m(i, j) * o->k_m(i, j + 1) / (o->k_m(i, j) + o->k_m(i, j + 1));
+            double s1 = (-o->T(i, j) + o->T(i, j + 1)) * o->rdy;
+            o->Ey(i, j) = s0 * s1;
+        }
+}
+
+// MixtureTemperatureKernel (bubble_CUDA_kernel.cuh:1036-1061): absorption of
+// the acoustic field, bubble heat and conduction, explicit in time; rho_m
+// and C_pm are the previous step's properties.
+void mixture_temperature(oracle *o) {
+    const ArrayIndex &a = o->a;
+    for (int j = a.jstam; j <= a.jendm; j++)
+        for (int i = a.istam; i <= a.iendm; i++) {
+            double s0 = (o->px(i, j) + o->py(i, j) - o->pnx(i, j) - o->pny(i, j)) / o->dt;
+            s0 = 2.0 * (0.22) * 0.115129255 / (o->rho_inf * o->cs_inf * o->omega * o->omega) * s0 * s0;
+            double Q = o->Work(i, j) + s0;
+            double s1 = (o->Ex(i, j) - o->Ex(i - 1, j)) * o->rdx;
+            double s2 = (o->Ey(i, j) - o->Ey(i, j - 1)) * o->rdy;
+            o->Work(i, j) = o->dt * (s1 + s2 + Q) / (o->rho_m(i, j) * o->C_pm(i, j));
+            o->T(i, j) += o->Work(i, j);
+        }
+}
+
+// MixtureBoundaryTemperatureKernel (bubble_CUDA_kernel.cuh:1064-1102), its
+// four parts in program order (the reference runs them in one grid whose
+// blocks race on the corner cells).  The y1 part starts at row Y, so row Y
+// is overwritten with the ghost row Y + 1, as in the reference.
+void mixture_boundary_temperature(oracle *o) {
+    const ArrayIndex &a = o->a;
+    const int X = o->cfg.grid_x, Y = o->cfg.grid_y;
+    if (!o->cfg.pml_x0)
+        for (int j = a.jstam; j <= a.jendm; j++)
+            for (int i = 1 + a.ns + a.ms; i <= 0; i++) o->T(i, j) = o->T(1 - i, j);
+    if (!o->cfg.pml_x1)
+        for (int j = a.jsta1m; j <= a.jend1m; j++)
+            for (int i = X + 1; i <= X + a.ne + a.me; i++) o->T(i, j) = o->T(X - (i - X - 1), j);
+    if (!o->cfg.pml_y0)
+        for (int i = a.ista2m; i <= a.iend2m; i++)
+            for (int j = 1 + a.ns + a.ms; j <= 0; j++) o->T(i, j) = o->T(i, 1 - j);
+    if (!o->cfg.pml_y1)
+        for (int i = a.ista2m; i <= a.iend2m; i++)
+            for (int j = Y; j <= Y + a.ne + a.me; j++) o->T(i, j) = o->T(i, Y - (j - Y - 1));
+}
+
+void calculate_temperature(oracle *o) {
+    std::fill(o->Work.v.begin(), o->Work.v.end(), 0.0);
+    if (o->cfg.bub_enable) bubble_heat(o);
+    mixture_km(o);
+    mixture_energy(o);
+    mixture_temperature(o);
+    mixture_boundary_temperature(o);
+}
+
 // MixturePropertiesKernel (bubble_CUDA_kernel.cuh:1110-1125)
 void calculate_properties(oracle *o) {
     const ArrayIndex &a = o->a;
@@ -1226,6 +1322,7 @@ int oracle_step(oracle *o, int32_t k, brp_step_stats *stats) {
             st.n_pic = loop;
             if (o->sched_mode == 1) o->sched_npic.push_back(loop);
         }
+        if (o->cfg.thermal) calculate_temperature(o);
         calculate_properties(o);
         if (status == BRP_OK && !finite_fields(o)) status = BRP_ERR_NONFINITE;
         st.nstep = o->nstep;

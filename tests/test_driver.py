@@ -123,4 +123,6 @@ def test_debug_p_min_key_cannot_be_set(tmp_path):
 def test_extension_keys(tmp_path):
     o = dry(text="[extensions]\nprecision = mixed\nsource_in_pml = 1\nfocus_x = 0.0125\n"
                  "rp_exact = yes\nmax_picard = 7\n", tmp_path=tmp_path)
-    assert o["extensions"] == [1, 1, 1, 0.0125, 7, 0, 1]
+    assert o["extensions"] == [1, 1, 1, 0.0125, 7, 0, 1, 1]
+    o = dry(text="[extensions]\nthermal = 0\n", tmp_path=tmp_path)
+    assert o["extensions"][-1] == 0

@@ -74,7 +74,7 @@ def conf_as_config():
              tr_amp=0.6, tr_freq=1.0e6, tr_focal_dist=3.4e-3, tr_box_size=0.6e-3, tr_on_wave=1000,
              tr_focused_p=1, bub_enable=1, bub_fg0=1e-6, bub_R0=3e-6, bub_P0=101325,
              bub_T0=293.15, bub_gamma=1.4, bub_sigma=0.0725, bub_mu=1e-3, bub_rho=1000,
-             bub_K0=26.2e-3, ext_focus_x=2.4e-3, ext_focus_x_set=1)
+             bub_K0=26.2e-3, ext_focus_x=2.4e-3, ext_focus_x_set=1, thermal=1)
     return c
 
 
@@ -106,7 +106,7 @@ def oracle_at(steps):
     for s in steps:
         o.step(s - done)
         done = s
-        out[s] = {n: o.field(n) for n in ("p0", "fg", "vx", "vy")}
+        out[s] = {n: o.field(n) for n in ("p0", "fg", "vx", "vy", "T")}
         out[s]["R_t"] = o.bubbles("R_t")
         out[s]["x"] = o.bubbles("x")
         out[s]["time"] = o.info()["time"]
@@ -135,6 +135,10 @@ def test_driver_ascii_rawdata(tmp_path):
         # vx is stored from i = -1: column i sits at index i + 1
         want_vx = ref[s]["vx"][:Y + 1, 1:X + 2]
         np.testing.assert_allclose(vx, want_vx, rtol=5e-6, atol=1e-6 * np.abs(want_vx).max())
+        T, _, _ = read_tuples(out / f"T_step_{s:05d}.txt", X, Y)
+        want_T = ref[s]["T"][:Y + 1, :X + 1]
+        assert np.abs(want_T).max() > 0
+        np.testing.assert_allclose(T, want_T, rtol=5e-6, atol=1e-6 * np.abs(want_T).max())
         br = np.loadtxt(out / f"bubble_radii_{s:05d}.txt")
         np.testing.assert_allclose(br[:, 2], ref[s]["R_t"], rtol=5e-6)
         np.testing.assert_allclose(br[:, 0], ref[s]["x"], rtol=5e-6)

@@ -238,3 +238,36 @@ def test_golden_fixture(brp):
         assert rel_l2(g.field("p0"), z["p0"]) < tol
         assert rel_l2(g.field("fg"), z["fg"]) < tol
         assert rel_l2(g.bubbles("R_t"), z["R_t"]) < tol
+
+
+def _mirror_case():
+    cfg = cases.sample_like_case(40)
+    cfg.update(grid_y=40, grid_ly=40e-4, pml_x1=0, pml_y1=0, tr_focused_p=0, tr_plane_p=1,
+               tr_plane_v=1, tr_focal_dist=3e-3, tr_box_size=1.0e-3, tr_cylindrical=0)
+    return cfg, None
+
+
+@pytest.mark.parametrize("case,prec,tol", [
+    ("mirrors", 0, 1e-11),       # all four temperature boundary parts, absorption, bubble heat
+    ("bubbly", 0, 1e-8),         # cavitating bubbles: Q_B from the fast RP chain
+    ("bubbly", 1, 1e-3),         # mixed: the absorption term squares FP32 pressure rates
+    ("cylindrical", 0, 1e-9),    # bubble heat / x
+])
+def test_temperature(brp, case, prec, tol):
+    """calculate_temperature (bubble_CUDA_kernel.cuh:955-1102,1347-1380): the
+    T field after 120 steps against the oracle."""
+    if case == "mirrors":
+        cfg, b = _mirror_case()
+    elif case == "bubbly":
+        cfg, b = cases.bubbly_case(64, nb=160, precision=prec)
+    else:
+        cfg, b = cases.sample_like_case(48, cylindrical=1), None
+    cfg.update(thermal=1, precision=prec)
+    g, o, gs, os_ = run_pair(brp, cfg, b, steps=120)
+    T_g, T_o = g.field("T"), o.field("T")
+    assert np.abs(T_o).max() > 0
+    err = rel_l2(T_g, T_o)
+    print(f"T {case} prec={prec}: rel L2 {err:.2e}, max {np.abs(T_o).max():.3e} K", flush=True)
+    assert err < tol
+    if prec == 0:  # the temperature is one-way: the acoustic fields are unchanged
+        assert rel_l2(g.field("p0"), o.field("p0")) < (1e-12 if b is None else 1e-9)

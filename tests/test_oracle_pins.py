@@ -414,3 +414,34 @@ def test_picard_skipped_on_first_step():
     assert st[0]["n_pic"] == 0 and st[0]["resimax0"] == 0.0
     np.testing.assert_array_equal(o.bubbles("R_p"), b[2])
     np.testing.assert_array_equal(o.bubbles("R_t"), b[2])
+
+
+def test_conduction_stencil_single_hot_cell():
+    """One explicit conduction step of the temperature solve from a single
+    hot cell, no sources: the centre loses 4 q and each face neighbour gains
+    q = dt k_h (T0 / dx^2) / (rho c_p), with k_h the harmonic mean of the
+    water conductivity at T_inf + T0 and T_inf (MixtureEnergyKernel /
+    MixtureTemperatureKernel, bubble_CUDA_kernel.cuh:1009-1061); nothing
+    else changes."""
+    c = cases.acoustic_case(24, amp_pa=0.0)
+    c.update(thermal=1)
+    o = cases.make(ol.OracleSim, c)
+    T = o.field("T")
+    ic, jc, T0 = 12, 11, 1.0
+    T[jc, ic] = T0
+    o.set_field("T", T)
+    o.step(1)
+    d = o.field("T") - T
+    T_inf, dx = 293.15, c.grid_lx / c.grid_x
+    dt = o.info()["dt"]
+    f = lambda v: float(np.float32(v))  # the reference's single-precision literals
+    k = lambda t: 0.76760 + f(7.5390e-3) * t - f(9.8250e-6) * t * t
+    cp = (917.5 - f(10.1016) * T_inf + f(0.0454134) * T_inf**2 - 9.07517e-5 * T_inf**3
+          + f(6.80700e-8) * T_inf**4) / f(18.0e-3)
+    kh = 2 * k(T_inf + T0) * k(T_inf) / (k(T_inf + T0) + k(T_inf))
+    q = dt * kh * T0 / dx**2 / (1000.0 * cp)
+    assert d[jc, ic] == pytest.approx(-4 * q, rel=1e-12)
+    for (j, i) in ((jc + 1, ic), (jc - 1, ic), (jc, ic + 1), (jc, ic - 1)):
+        assert d[j, i] == pytest.approx(q, rel=1e-12)
+    d[jc, ic] = d[jc + 1, ic] = d[jc - 1, ic] = d[jc, ic + 1] = d[jc, ic - 1] = 0.0
+    assert np.abs(d).max() == 0.0
