@@ -59,6 +59,7 @@ def load_library():
         "brp_save_state": (C.c_int, [vp, i32]),
         "brp_restore_state": (C.c_int, [vp, i32]),
         "brp_get_phase_times": (C.c_int, [vp, dp]),
+        "brp_set_focus": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "brp_adjoint": (C.c_int, [vp, C.POINTER(abi.Objective), dp, dp, dp, i64,
                                   C.POINTER(abi.AdjointStats)]),
     }
@@ -124,6 +125,9 @@ class _Backend:
         if check:
             self._check(rc)
         return out
+
+    def set_focus(self, fp_x, fp_y, f_dist):
+        self._check(self._f("set_focus")(self.h, fp_x, fp_y, f_dist))
 
     def shape(self, name):
         fid = abi.FIELD_NAMES[name] if isinstance(name, str) else name

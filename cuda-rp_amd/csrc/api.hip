@@ -33,6 +33,7 @@ struct Snapshot {
     // scalar state
     int pc, fcur, fn, fm, fz, rp_cur, pl_m, pl_n, pl_p, first_step;
     Window W;
+    double ctrl_x, ctrl_y, ctrl_dist;  // source focus (moved by brp_set_focus)
     int64_t nstep;
     double t, tx;
 };
@@ -476,6 +477,9 @@ int snap_save(brp_handle *h, Snapshot &sn) {
     sn.pl_p = h->pl_p;
     sn.first_step = h->first_step;
     sn.W = h->W;
+    sn.ctrl_x = h->ctrl_x;
+    sn.ctrl_y = h->ctrl_y;
+    sn.ctrl_dist = h->ctrl_dist;
     sn.nstep = h->nstep;
     sn.t = h->t;
     sn.tx = h->tx;
@@ -499,6 +503,12 @@ int snap_restore(brp_handle *h, const Snapshot &sn) {
     h->pl_p = sn.pl_p;
     h->first_step = sn.first_step;
     h->W = sn.W;
+    if (h->ctrl_x != sn.ctrl_x || h->ctrl_y != sn.ctrl_y || h->ctrl_dist != sn.ctrl_dist) {
+        h->ctrl_x = h->g.fp_x = sn.ctrl_x;
+        h->ctrl_y = h->g.fp_y = sn.ctrl_y;
+        h->ctrl_dist = h->g.fdist = sn.ctrl_dist;
+        find_aperture(h);
+    }
     h->nstep = sn.nstep;
     h->t = sn.t;
     h->tx = sn.tx;
@@ -762,6 +772,9 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                   h->heat_d, h->fgb[prev], h->fgb[old_cur], h->first_step,
                                   (const F *)h->pb[p_i][0], (const F *)h->pb[p_i][1],
                                   (const F *)h->pb[pn_i][0], (const F *)h->pb[pn_i][1]);
+            launch_focal_point(s, h->g, h->dT_d, h->heat_d, h->sc_d);
+            HIPCHK(h, hipMemcpyAsync(h->sc_h->focal, h->sc_d->focal, sizeof(double) * 2,
+                                     hipMemcpyDeviceToHost, s));
             HIPCHK(h, hipEventRecord(h->ev[4], s));
         }
         // rotate the void-fraction buffers: final iterate in `prev`, zero buffer in `target`
@@ -798,6 +811,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         st.time = h->t;
         st.resimax = resimax;
         st.n_pic = loop;
+        if (h->cfg.thermal) {
+            st.focal_x = h->sc_h->focal[0];
+            st.focal_y = h->sc_h->focal[1];
+        }
         if (rec) {
             rec->n_pic = loop;
             if (rec->want_grid) {
@@ -1572,6 +1589,17 @@ int brp_adjoint(brp_handle *h, const brp_objective *obj, double *grad_amp, doubl
     return h->prec == BRP_PREC_FP64
                ? adjoint_impl<double>(h, o, grad_amp, grad_phase, grad_fg0, ld, stats)
                : adjoint_impl<float>(h, o, grad_amp, grad_phase, grad_fg0, ld, stats);
+}
+
+int brp_set_focus(brp_handle *h, double fp_x, double fp_y, double f_dist) {
+    if (!h) return BRP_ERR_INVALID;
+    if (!std::isfinite(fp_x) || !std::isfinite(fp_y) || !std::isfinite(f_dist) || f_dist < 0)
+        return set_err(h, BRP_ERR_INVALID, "set_focus: finite point and f_dist >= 0 required");
+    h->ctrl_x = h->g.fp_x = fp_x;
+    h->ctrl_y = h->g.fp_y = fp_y;
+    h->ctrl_dist = h->g.fdist = f_dist;
+    find_aperture(h);
+    return BRP_OK;
 }
 
 int brp_get_phase_times(const brp_handle *h, double ms[5]) {

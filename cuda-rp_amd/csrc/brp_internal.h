@@ -62,8 +62,11 @@ struct StepScalars {
                                          // range / nonfinite R, bit 2 sub-step cap
     int sub_max;                         // max RP sub-steps this step
     unsigned long long sub_sum;          // sum of RP sub-steps this step
+    double focal[2];                     // (x, y) of the largest temperature increment
     unsigned long long resid[kMaxEval];  // max |dp0| bit patterns (>= 0 doubles)
     unsigned int work[kMaxEval];         // bubble work-queue heads, one per Picard iteration
+    double focal_v[256];                 // per-block maxima of the focal-point search
+    int focal_i[256];
 };
 
 // Bubble arrays (device pointers; n padded to a multiple of 256).
@@ -144,6 +147,9 @@ void launch_temperature(hipStream_t s, const GridParams &g, const BubbleParams &
                         const BubbleArrays &b, bool bubbles, Window w, double T_inf, double *T,
                         double *dT, double *heat, const int64_t *fg, const int64_t *fgn,
                         int first, const F *px, const F *py, const F *pnx, const F *pny);
+
+void launch_focal_point(hipStream_t s, const GridParams &g, const double *dT, const double *heat,
+                        StepScalars *sc);
 
 // ---- adjoint (kernels_adjoint.hip) ----
 // Adjoint fields, FP64, same frame as the forward fields.

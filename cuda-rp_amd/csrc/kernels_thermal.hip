@@ -13,6 +13,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace brp {
 namespace {
 
@@ -149,7 +151,73 @@ __global__ __launch_bounds__(1024) void k_temperature_ghosts(GridParams g, doubl
         }
 }
 
+// determine_focal_point (bubble_CUDA_kernel.cuh:1134-1166) on the reference's
+// Work array after calculate_temperature: the temperature increment in the
+// interior, the bubble heat deposited in the ghost cells.  The reference's
+// reduction functors are not in its tree; here the focal point is the cell
+// of the largest value (the first in row-major order on ties) at
+// (x, y) = (i dx, j dy), the coordinates of the reference's output files.
+__device__ inline void better(double &v, int &k, double v2, int k2) {
+    if (v2 > v || (v2 == v && k2 < k)) {
+        v = v2;
+        k = k2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_focal_blocks(GridParams g, const double *dT,
+                                                      const double *heat, StepScalars *sc) {
+    const int W = g.N + 2, n = W * (g.M + 2);
+    double v = -INFINITY;
+    int k = 0x7fffffff;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int i = q % W, j = q / W;
+        const int64_t o = frame_off(i, j, g.pitch);
+        const bool in = i >= 1 && i <= g.N && j >= 1 && j <= g.M;
+        better(v, k, in ? dT[o] : (heat ? heat[o] : 0.0), q);
+    }
+    __shared__ double sv[256];
+    __shared__ int sk[256];
+    sv[threadIdx.x] = v;
+    sk[threadIdx.x] = k;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) better(sv[threadIdx.x], sk[threadIdx.x], sv[threadIdx.x + s], sk[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sc->focal_v[blockIdx.x] = sv[0];
+        sc->focal_i[blockIdx.x] = sk[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_focal_final(GridParams g, int nblk, StepScalars *sc) {
+    __shared__ double sv[256];
+    __shared__ int sk[256];
+    const int t = threadIdx.x;
+    sv[t] = t < nblk ? sc->focal_v[t] : -INFINITY;
+    sk[t] = t < nblk ? sc->focal_i[t] : 0x7fffffff;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) better(sv[t], sk[t], sv[t + s], sk[t + s]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int W = g.N + 2;
+        const int k = sk[0] == 0x7fffffff ? 0 : sk[0];
+        sc->focal[0] = (double)(k % W) * g.dx;
+        sc->focal[1] = (double)(k / W) * g.dy;
+    }
+}
+
 }  // namespace
+
+void launch_focal_point(hipStream_t s, const GridParams &g, const double *dT, const double *heat,
+                        StepScalars *sc) {
+    const int n = (g.N + 2) * (g.M + 2);
+    const int nblk = std::min(256, (n + 255) / 256);
+    hipLaunchKernelGGL(k_focal_blocks, dim3(nblk), dim3(256), 0, s, g, dT, heat, sc);
+    hipLaunchKernelGGL(k_focal_final, dim3(1), dim3(256), 0, s, g, nblk, sc);
+}
 
 template <typename F>
 void launch_temperature(hipStream_t s, const GridParams &g, const BubbleParams &bp,

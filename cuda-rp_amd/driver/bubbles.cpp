@@ -11,8 +11,11 @@
 //     shell, output_styles.cpp:19-31);
 //   * snapshots are written by one writer thread from copies, so the min/max
 //     files are not raced (output_styles.cpp:69-100, 199-247).
-// Focal-point tracking and the PID controller (bubble_CUDA.cu:310-352) are not
-// part of this build: the focus stays where the configuration puts it.
+// Focal-point tracking and the PID controller (bubble_CUDA.cu:310-352) run
+// here on the host, as in the reference: the library reports each step's raw
+// focal point (the cell of the largest temperature increment), the driver
+// filters it, runs the controller and moves the source focus for the next
+// step with brp_set_focus.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -31,6 +34,51 @@ using namespace brp_driver;
 
 namespace {
 
+// 4th-order Butterworth low-pass filter (filter_loop,
+// bubble_CUDA_kernel.cuh:118-135).  Its state is process-wide, as the
+// reference's file-scope statics, so it carries over between passes.
+struct Butterworth {
+    double xv[5] = {0, 0, 0, 0, 0}, yv[5] = {0, 0, 0, 0, 0};
+    double operator()(double input) {
+        const double gain = 6.522112693e+08;
+        for (int k = 0; k < 4; k++) {
+            xv[k] = xv[k + 1];
+            yv[k] = yv[k + 1];
+        }
+        xv[4] = input / gain;
+        yv[4] = (xv[0] + xv[4]) + 4 * (xv[1] + xv[3]) + 6 * xv[2] + (-0.9676955438 * yv[0]) +
+                (3.9025587848 * yv[1]) + (-5.9020258615 * yv[2]) + (3.9671625959 * yv[3]);
+        return yv[4];
+    }
+};
+Butterworth g_filter;
+
+struct Vec2 {
+    double x = 0.0, y = 0.0;
+};
+
+// The controller state handed from one pass to the next (solve_bubbles'
+// returned tuple, bubble_CUDA.cu:470-475).
+struct PidCarry {
+    bool valid = false;
+    Vec2 control, err_total, d_err;
+};
+
+// focal_PID (bubble_CUDA_kernel.cuh:1168-1184), componentwise.
+Vec2 focal_pid(Vec2 target, Vec2 actual, Vec2 prev, Vec2 *d_err, Vec2 *err_total) {
+    const double K = 0.2 * 0.60;
+    const double Pc = 1000.0;
+    const double D = K * 0.125 * Pc;
+    const double I = K / (0.5 * Pc);
+    const Vec2 err{target.x - actual.x, target.y - actual.y};
+    err_total->x = err_total->x + err.x;
+    err_total->y = err_total->y + err.y;
+    d_err->x = prev.x - actual.x;
+    d_err->y = prev.y - actual.y;
+    return Vec2{target.x + (K * err.x + D * d_err->x + I * err_total->x),
+                target.y + (K * err.y + D * d_err->y + I * err_total->y)};
+}
+
 bool get(brp_handle *h, int id, Field *f) {
     int nx, ny, i0, j0;
     if (brp_field_shape(h, id, &nx, &ny, &i0, &j0) != BRP_OK) return false;
@@ -47,16 +95,17 @@ int fail(brp_handle *h, const char *what) {
     return 1;
 }
 
-// One pass of solve_bubbles (bubble_CUDA.cu:56-476).
+// One pass of solve_bubbles (bubble_CUDA.cu:56-476).  focalpoint / control
+// receive the pass's filtered focal points and controller outputs.
 int run_pass(const Options &o, const std::string &dir, RawWriter **writer_out, int *steps_done,
-             double *focus_y) {
+             PidCarry *pid, std::vector<Vec2> *focalpoint, std::vector<Vec2> *control) {
     brp_handle *h = nullptr;
     std::printf("Computing Simulation Variables...");
     if (brp_create(&o.cfg, &h) != BRP_OK) return fail(nullptr, "create");
     std::printf("\tdone\n");
-    if (o.cfg.tr_pid)
-        std::printf("Note: focal-point tracking and PID control are not available in this "
-                    "build; the focus stays fixed.\n");
+    if (o.cfg.tr_pid && !o.cfg.thermal)
+        std::printf("Note: focal-point tracking needs the temperature solve "
+                    "([extensions] thermal = 1); the focus stays fixed.\n");
     std::printf("Preparing folders...");
     std::string err;
     if (!prepare_directory(dir, &err)) {
@@ -74,8 +123,26 @@ int run_pass(const Options &o, const std::string &dir, RawWriter **writer_out, i
     std::printf("\tdone\n");
     brp_info info;
     brp_get_info(h, &info);
-    *focus_y = info.fp_y;
     Geometry geo{o.cfg.grid_x, o.cfg.grid_y, info.dx, info.dy, o.cfg.tr_cylindrical != 0};
+    // the transducer's focal point fp (init_transducer) and the initial
+    // control point (bubble_CUDA.cu:125-153)
+    const Vec2 fp{info.fp_x, info.fp_y};
+    focalpoint->assign(1, Vec2{0.0, 0.0});
+    control->clear();
+    Vec2 err_total, d_err;
+    if (o.cfg.tr_pid && pid->valid) {
+        control->push_back(pid->control);
+        err_total = pid->err_total;
+        d_err = pid->d_err;
+    } else if (o.cfg.tr_pid && o.cfg.tr_init_control) {
+        control->push_back(Vec2{fp.x, o.cfg.tr_init_control});
+    } else {
+        control->push_back(fp);
+    }
+    const bool tracking = o.cfg.thermal != 0;
+    if (o.cfg.tr_pid && pid->valid)  // later passes start from the previous pass's control
+        brp_set_focus(h, control->back().x, control->back().y,
+                      control->back().y / 0.5 / std::sqrt(3.0));
     const int save_function = (o.ascii ? 1 : 0) | (o.sph ? 2 : 0);
     RawWriter *writer = *writer_out;
     if (!writer) writer = new RawWriter(dir, geo, (save_function & 2) != 0);
@@ -96,6 +163,24 @@ int run_pass(const Options &o, const std::string &dir, RawWriter **writer_out, i
         }
         nstep = (int)st.nstep;
         tstep = st.time;
+        if (tracking) {
+            // bubble_CUDA.cu:310-348: filtered focal y, then the controller
+            focalpoint->push_back(Vec2{0.0, g_filter(st.focal_y)});
+            if (o.cfg.tr_pid && nstep > o.cfg.tr_pid_start_step) {
+                const size_t k = focalpoint->size();
+                Vec2 u = focal_pid(fp, (*focalpoint)[k - 1], (*focalpoint)[k - 2], &d_err,
+                                   &err_total);
+                u.x = fp.x;  // the reference sets 0, its fp.x without the focus_x extension
+                u.y = std::min(std::max(u.y, fp.y), o.cfg.grid_ly);
+                control->push_back(u);
+                if (brp_set_focus(h, u.x, u.y, u.y / 0.5 / std::sqrt(3.0)) != BRP_OK) {
+                    int rc = fail(h, "set_focus");
+                    writer->finish();
+                    brp_destroy(h);
+                    return rc;
+                }
+            }
+        }
         if (c.sim_n_step_max != 0)
             std::printf("\rRunning the simulation...\t\tnstep : %5i / %i\t", nstep, c.sim_n_step_max);
         else
@@ -136,6 +221,10 @@ int run_pass(const Options &o, const std::string &dir, RawWriter **writer_out, i
     }
     std::printf("\rRunning the simulation...\t\tdone\n\n");
     *steps_done = nstep;
+    pid->valid = true;
+    pid->control = control->back();
+    pid->err_total = err_total;
+    pid->d_err = d_err;
     brp_destroy(h);
     return 0;
 }
@@ -178,6 +267,7 @@ int main(int argc, char **argv) {
 
     double fg_min = 1e10, fg_max = -1e10, p0_min = 1e10, p0_max = -1e10, T_min = 1e10, T_max = -1e10;
     std::string target_dir = o.target_dir;
+    PidCarry pid;
     for (int j = 0; j < o.num_iters; j++) {
         // multi-pass runs get directory_I/, directory_II/, ... (bubbles.cpp:224-246)
         if (o.num_iters > 1) {
@@ -187,8 +277,8 @@ int main(int argc, char **argv) {
         }
         RawWriter *writer = nullptr;
         int steps = 0;
-        double focus_y = 0.0;
-        int rc = run_pass(o, target_dir, &writer, &steps, &focus_y);
+        std::vector<Vec2> focalpoint, control;
+        int rc = run_pass(o, target_dir, &writer, &steps, &pid, &focalpoint, &control);
         if (writer) {
             writer->finish();
             fg_min = std::min(fg_min, writer->fg_min);
@@ -205,12 +295,12 @@ int main(int argc, char **argv) {
             }
         }
         if (rc) return rc;
-        // focal point files (bubbles.cpp:258-270): without focal tracking the
-        // actual series holds its initial entry and the controller the fixed focus
+        // focal point files (bubbles.cpp:258-270)
         std::ofstream fa(target_dir + "focalpoint_actual_T.txt");
-        fa << 0 << "\t" << 0.0 << std::endl;
+        for (size_t i = 0; i < focalpoint.size(); i++) fa << i << "\t" << focalpoint[i].y << std::endl;
         std::ofstream fc(target_dir + "focalpoint_controller_T.txt");
-        fc << 0 + o.cfg.tr_pid_start_step << "\t" << focus_y << std::endl;
+        for (size_t i = 0; i < control.size(); i++)
+            fc << i + o.cfg.tr_pid_start_step << "\t" << control[i].y << std::endl;
         if (o.dbg_T) {
             std::ofstream f(target_dir + "T_minmax.txt");
             f << (o.has_T_min ? o.T_min : T_min) << std::endl << (o.has_T_max ? o.T_max : T_max) << std::endl;

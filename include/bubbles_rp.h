@@ -124,6 +124,11 @@ typedef struct brp_step_stats {
     double rp_mean_substeps;     /* mean RP sub-steps per bubble per call    */
     int32_t status;              /* BRP_OK or the error of this step         */
     int32_t pad_;
+    /* With thermal = 1: the cell (x = i dx, y = j dy) of the largest
+     * temperature increment of the step over the 2m grid, the raw input of
+     * the reference's focal-point tracking (determine_focal_point on Work
+     * after calculate_temperature, bubble_CUDA.cu:310-324); 0 otherwise. */
+    double focal_x, focal_y;
 } brp_step_stats;
 
 /* Derived scalars (init_grid_size, mix_set_time_increment, init_bub_params). */
@@ -245,6 +250,13 @@ int brp_get_bubbles(brp_handle *h, int32_t which, double *dst);
  * bit.  No reference counterpart (the reference has no checkpointing). */
 int brp_save_state(brp_handle *h, int32_t slot);
 int brp_restore_state(brp_handle *h, int32_t slot);
+
+/* Moves the focused source's control point and focal distance (the per-step
+ * focal_point_c / focal_dist_c upload of bubble_CUDA.cu:213-215, driven by
+ * the PID controller of bubble_CUDA.cu:326-348) for the following steps; the
+ * aperture (the row-0 cells within f_dist of the point) and its element
+ * partition are recomputed. */
+int brp_set_focus(brp_handle *h, double fp_x, double fp_y, double f_dist);
 
 /* Device time of the last brp_step call split by phase (ms): [0] velocity,
  * [1] bubble motion, [2] pressure, [3] gather+RP+scatter, [4] other (ordering of the bubble work queue). */

@@ -1322,7 +1322,24 @@ int oracle_step(oracle *o, int32_t k, brp_step_stats *stats) {
             st.n_pic = loop;
             if (o->sched_mode == 1) o->sched_npic.push_back(loop);
         }
-        if (o->cfg.thermal) calculate_temperature(o);
+        if (o->cfg.thermal) {
+            calculate_temperature(o);
+            // determine_focal_point (bubble_CUDA_kernel.cuh:1134-1166) on Work:
+            // its reduction functors are not in the reference tree; the
+            // largest value, first in row-major order, at (i dx, j dy)
+            const ArrayIndex &a = o->a;
+            double best = -INFINITY;
+            int bi = a.ista2m, bj = a.jsta2m;
+            for (int j = a.jsta2m; j <= a.jend2m; j++)
+                for (int i = a.ista2m; i <= a.iend2m; i++)
+                    if (o->Work(i, j) > best) {
+                        best = o->Work(i, j);
+                        bi = i;
+                        bj = j;
+                    }
+            st.focal_x = bi * o->dx;
+            st.focal_y = bj * o->dy;
+        }
         calculate_properties(o);
         if (status == BRP_OK && !finite_fields(o)) status = BRP_ERR_NONFINITE;
         st.nstep = o->nstep;
@@ -1339,6 +1356,15 @@ int oracle_step(oracle *o, int32_t k, brp_step_stats *stats) {
             return status;
         }
     }
+    return BRP_OK;
+}
+
+int oracle_set_focus(oracle *o, double fp_x, double fp_y, double f_dist) {
+    if (!o) return BRP_ERR_INVALID;
+    o->control_x = fp_x;
+    o->control_y = fp_y;
+    o->control_dist = f_dist;
+    find_aperture(o);
     return BRP_OK;
 }
 

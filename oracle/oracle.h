@@ -48,6 +48,8 @@ const char *oracle_last_error(const oracle *o);
  * replay position; mode 1 clears the record.  schedule_copy hands a record to
  * another instance with the same bubble count. */
 int oracle_schedule_mode(oracle *o, int32_t mode);
+/* brp_set_focus of the oracle. */
+int oracle_set_focus(oracle *o, double fp_x, double fp_y, double f_dist);
 int oracle_schedule_copy(oracle *dst, const oracle *src);
 /* OpenMP threads used by oracle_step (1 = serial; result is identical). */
 void oracle_set_threads(oracle *o, int32_t n);

@@ -63,6 +63,7 @@ def load():
         "oracle_rp_radius": (None, [vp, C.c_double, C.c_double, dp]),
         "oracle_schedule_mode": (C.c_int, [vp, i32]),
         "oracle_schedule_copy": (C.c_int, [vp, vp]),
+        "oracle_set_focus": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -162,3 +162,62 @@ def test_driver_sph_rawdata(tmp_path):
     np.testing.assert_allclose(data, want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
     assert not list(out.glob("*.txt")) or all("minmax" in p.name or "focal" in p.name or
                                               "runtime" in p.name for p in out.glob("*.txt"))
+
+
+def _butterworth():
+    xv, yv = [0.0] * 5, [0.0] * 5
+
+    def f(u):
+        xv[:4] = xv[1:]
+        yv[:4] = yv[1:]
+        xv[4] = u / 6.522112693e+08
+        yv[4] = ((xv[0] + xv[4]) + 4 * (xv[1] + xv[3]) + 6 * xv[2] + (-0.9676955438 * yv[0])
+                 + (3.9025587848 * yv[1]) + (-5.9020258615 * yv[2]) + (3.9671625959 * yv[3]))
+        return yv[4]
+    return f
+
+
+def test_driver_focal_tracking_and_pid(tmp_path):
+    """transducer.pid = 1: the driver's focal-point series (filtered) and the
+    controller's outputs follow the same loop run on the oracle
+    (bubble_CUDA.cu:310-348, focal_PID and filter_loop of
+    bubble_CUDA_kernel.cuh:118-135,1168-1184)."""
+    conf = tmp_path / "conf.txt"
+    conf.write_text(CONF.replace("focal_dist = 3.4e-3\n", "focal_dist = 3.4e-3\npid = 1\n"
+                                 "pid_start_step = 10\n"))
+    out = tmp_path / "raw"
+    r = subprocess.run([BIN, "-a", "-d", str(out), str(conf)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fa = np.loadtxt(out / "focalpoint_actual_T.txt")
+    fc = np.loadtxt(out / "focalpoint_controller_T.txt")
+    # the same loop on the oracle
+    c = conf_as_config()
+    c.update(tr_pid=1, tr_pid_start_step=10)
+    o = ol.OracleSim(c)
+    o.prepare()
+    info = o.info()
+    fp = (info["fp_x"], info["fp_y"])
+    filt = _butterworth()
+    focal, control = [0.0], [fp[1]]
+    err_tot, d_err = [0.0, 0.0], [0.0, 0.0]
+    K, Pc = 0.2 * 0.60, 1000.0
+    D, I = K * 0.125 * Pc, K / (0.5 * Pc)
+    for n in range(1, 61):
+        st = o.step(1)[0]
+        focal.append(filt(st["focal_y"]))
+        if n > 10:
+            act, prev = (0.0, focal[-1]), (0.0, focal[-2])
+            err = [fp[0] - act[0], fp[1] - act[1]]
+            err_tot = [err_tot[0] + err[0], err_tot[1] + err[1]]
+            d_err = [prev[0] - act[0], prev[1] - act[1]]
+            uy = fp[1] + (K * err[1] + D * d_err[1] + I * err_tot[1])
+            uy = min(max(uy, fp[1]), c.grid_ly)
+            control.append(uy)
+            o.set_focus(fp[0], uy, uy / 0.5 / np.sqrt(3.0))
+    assert fa.shape == (61, 2) and fc.shape == (51, 2)
+    np.testing.assert_array_equal(fa[:, 0], np.arange(61))
+    np.testing.assert_array_equal(fc[:, 0], np.arange(51) + 10)
+    np.testing.assert_allclose(fa[:, 1], focal, rtol=5e-6, atol=1e-12)
+    np.testing.assert_allclose(fc[:, 1], control, rtol=5e-6)
+    assert len(set(np.round(fc[:, 1], 9))) > 1  # the controller moved the focus
