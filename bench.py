@@ -196,6 +196,32 @@ def grid_roofline(ph, n, steps, mixed):
             "bytes_per_cell_update": per_cell}
 
 
+def thermal_leg(brp, abi, n, nb, bub, warmup, steps, other_without_ms):
+    """The same workload with the one-way temperature solve and the focal
+    search on (bubble heat scatter, conductivity/flux/increment pass, the add
+    pass, ghosts, argmax of the work field, one 16-byte read-back), timed over
+    `steps` steps after `warmup`.  Its device time is the "other" phase minus
+    the same phase without the thermal solve.  Algorithmic bytes per cell:
+    increment T, f_g, f_g(n), heat (8 each), p.x, p.y, pn.x, pn.y (4 each in
+    mixed mode), dT written (8); add T, dT read, T written (24); argmax dT
+    read (8): 88 bytes."""
+    cfg, _ = c2_workload(abi, abi.PREC_MIXED, n, nb)
+    cfg.thermal = 1
+    sim, st, wall, ph, _, _ = run_gpu(brp, cfg, bub, warmup, steps, 0)
+    sim.close()
+    th_ms = max(ph["other"] / steps - other_without_ms, 1e-9)
+    per_cell = 88.0
+    achieved = per_cell * n * n / (th_ms * 1e-3) / 1e9
+    return {"ms_per_step": wall / steps * 1e3, "thermal_ms_per_step": th_ms,
+            "steps_window": f"{warmup + 1}-{warmup + steps}",
+            "focal_y_last": st[-1]["focal_y"],
+            "roofline": {"kernels": "k_bubble_heat + k_temperature_increment + k_temperature_add "
+                                    "+ k_temperature_ghosts + k_focal_blocks/final",
+                         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_cell": per_cell}}
+
+
 def calibrate_copy(mib=1024, reps=5):
     """A plain device copy of known size (reads and writes `mib` MiB per
     launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale
@@ -224,6 +250,8 @@ def main():
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--bubbles", type=int, default=1 << 20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-thermal", action="store_true")
+    ap.add_argument("--thermal-steps", type=int, default=100)
     ap.add_argument("--adjoint-steps", type=int, default=4000)
     ap.add_argument("--adjoint-window", type=int, default=1000)
     ap.add_argument("--adjoint-checkpoint", type=int, default=200)
@@ -325,6 +353,12 @@ def main():
         log(f"  fp64: {wall_d / args.steps * 1e3:.3f} ms/step; p0 err {err_p0:.2e}")
         sim_d.close()
     sim_m.close()
+
+    if not args.no_thermal:
+        log("thermal leg")
+        out["thermal"] = thermal_leg(brp, abi, n, nb, bub, args.warmup, args.thermal_steps,
+                                     ph_m["other"] / args.steps)
+        log(f"  thermal: {out['thermal']['thermal_ms_per_step']:.3f} ms/step")
 
     if not args.no_adjoint:
         log("adjoint leg")

@@ -80,6 +80,7 @@ struct BubbleArrays {
     double *vx, *vy;      // v_B (= v_L)
     int *sub;             // sub-steps of the last RP call, per bubble
     int *order;           // work-queue order: bubbles with many sub-steps first
+    int *order_tmp;       // partition scratch: the extreme class before its sort
     int *part;            // partition scratch: per-block extreme and heavy offsets + totals
     int *bbox_blk;        // per-block stencil bounding boxes of k_motion (4 ints each)
 };

@@ -284,9 +284,14 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
                           4.0 * bp.pi * bp.K0,
                           16.0 * bp.pi * bp.mu};
     RPFastConst fk{0, 0, 0, 0, 0, 0, 0};
-    // extreme bubbles (the queue's first E entries) run one per workgroup in
-    // lane 0 of its first wavefront, so that their long sub-step chains do not
-    // share a wavefront with each other; the shared queue starts after them
+    // The E longest bubbles (the queue's first entries) run one per workgroup
+    // in lane 0 of its first wavefront, alone: a wavefront issues the same
+    // instructions whether one lane or 64 are active, so a lone lane skips
+    // the paths the other lanes would need (the coth of the thermal
+    // impedance is one multiply for s > 20), and it never waits for the
+    // other lanes' refills.  The wavefront runs at raised issue priority
+    // until that bubble is done, then joins the shared queue, which starts
+    // after the E dedicated entries.
     const int E = min(b.part[(b.n + 255) / 256], (int)gridDim.x);
     int idx = -1;                  // bubble of this lane, -1 when idle
     bool pre = false;
@@ -294,6 +299,8 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
         idx = b.order[blockIdx.x];
         pre = true;
     }
+    bool solo = threadIdx.x < 64 && (int)blockIdx.x < E;  // wave-uniform
+    if (solo) __builtin_amdgcn_s_setprio(3);
     int batch_next = 0, batch_end = 0;  // wave-uniform
     bool exhausted = false;             // wave-uniform
     RPLane L;              // operation-by-operation path
@@ -305,8 +312,12 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
     unsigned long long sub_sum = 0;
     bool bad = false, capped_any = false;
     while (true) {
+        if (solo && __ballot(idx >= 0) == 0ull) {
+            solo = false;
+            __builtin_amdgcn_s_setprio(0);
+        }
         // ---- refill idle lanes from the wave's batch / the global queue
-        bool need = idx < 0;
+        bool need = idx < 0 && !solo;
         unsigned long long m = __ballot(need);
         bool fresh = pre;
         pre = false;
@@ -510,14 +521,13 @@ void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &
                            dim3(256), 0, s, g, bp, b, cur, plm, pln, plp, p0, fg_target, sc, slot);
 }
 
-// Stable partition of the work queue: bubbles whose last RP call took at
-// least `threshold` sub-steps first (longest-first scheduling: their long
-// sequential sub-step chains start at once and overlap everything else),
-// each group in index (cell-sorted) order.  Three passes: per-block counts,
-// one-block exclusive scan, scatter.
-// Work-queue order: bubbles whose last RP call took >= ext sub-steps first
-// ("extreme": each gets a wavefront of its own at launch), then those with
-// >= thr ("heavy"), then the rest; stable within each class.
+// Work-queue order (longest-first scheduling: long sequential sub-step
+// chains start at once and overlap everything else): bubbles whose last RP
+// call took >= ext sub-steps ("extreme"), sorted by descending count, then
+// those with >= thr ("heavy"), then the rest, the last two in index
+// (cell-sorted) order.  Passes: per-block class counts, one-block exclusive
+// scans, scatter (the extreme class into a scratch array), then the sort of
+// the extreme class.
 __global__ __launch_bounds__(256) void k_part_count(const int *sub, int n, int ext, int thr,
                                                     int *cntE, int *cntH) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
@@ -581,7 +591,7 @@ __device__ inline int block_rank(bool f, int *wcnt, int *total) {
 
 __global__ __launch_bounds__(256) void k_part_scatter(const int *sub, int n, int ext, int thr,
                                                       const int *offE, const int *offH, int nblk,
-                                                      int *order) {
+                                                      int *order, int *ext_out) {
     __shared__ int wcnt[4];
     const int idx = blockIdx.x * 256 + threadIdx.x;
     const bool valid = idx < n;
@@ -595,11 +605,52 @@ __global__ __launch_bounds__(256) void k_part_scatter(const int *sub, int n, int
     const int bE = offE[blockIdx.x], bH = offH[blockIdx.x];
     if (valid) {
         if (e)
-            order[bE + re] = idx;
+            ext_out[bE + re] = idx;
         else if (h)
             order[totE + bH + rh] = idx;
         else  // light rank = position among the block's light bubbles
             order[totE + totH + (blockIdx.x * 256 - bE - bH) + (threadIdx.x - re - rh)] = idx;
+    }
+}
+
+// The extreme class, longest first: ext_in[0, E) (E = *totE) scattered into
+// order[0, E) by descending sub-step count, in bins of 8 sub-steps (order
+// within a bin unspecified; results do not depend on the queue order).  One
+// workgroup: bin histogram in LDS, exclusive scan, scatter with LDS atomics.
+constexpr int kSortBins = 2048;
+__device__ inline int sort_bin(int v) {  // ascending bin = descending count
+    return kSortBins - 1 - min(v >> 3, kSortBins - 1);
+}
+
+__global__ __launch_bounds__(1024) void k_sort_extreme(const int *sub, const int *totE,
+                                                       const int *ext_in, int *order) {
+    __shared__ int hist[kSortBins];
+    __shared__ int wsum[16];
+    const int E = *totE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < kSortBins; k += 1024) hist[k] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < E; i += 1024) atomicAdd(&hist[sort_bin(sub[ext_in[i]])], 1);
+    __syncthreads();
+    // exclusive scan: thread t owns bins 2t and 2t + 1
+    const int h0 = hist[2 * threadIdx.x], h1 = hist[2 * threadIdx.x + 1];
+    int x = h0 + h1;  // inclusive wave scan of the pair sums
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int before = 0;
+    for (int q = 0; q < w; q++) before += wsum[q];
+    const int excl = before + x - (h0 + h1);
+    __syncthreads();
+    hist[2 * threadIdx.x] = excl;
+    hist[2 * threadIdx.x + 1] = excl + h0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < E; i += 1024) {
+        const int idx = ext_in[i];
+        order[atomicAdd(&hist[sort_bin(sub[idx])], 1)] = idx;
     }
 }
 
@@ -612,7 +663,9 @@ void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold,
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntE, nblk);
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntH, nblk);
     hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
-                       cntE, cntH, nblk, b.order);
+                       cntE, cntH, nblk, b.order, b.order_tmp);
+    hipLaunchKernelGGL(k_sort_extreme, dim3(1), dim3(1024), 0, s, b.sub, cntE + nblk, b.order_tmp,
+                       b.order);
 }
 
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
