@@ -14,10 +14,16 @@
 //     u = alpha coth(alpha) - 1;
 //   * omega_N and 1/omega_N from one rsqrt of eta; sqrt(c_alpha omega / R)
 //     and its reciprocal from one rsqrt;
-//   * reciprocals by v_rcp_f64 and two Newton steps (drcp), step-size bounds
-//     by rsqrt; every fused multiply-add is written out (the file is compiled
-//     with contraction off), so the instruction sequence is fixed by the
-//     source.
+//   * reciprocals by v_rcp_f64 and two Newton steps (drcp), reciprocal
+//     square roots by v_rsq_f64 and one second-order step (drsqrt); every
+//     fused multiply-add is written out (the file is compiled with
+//     contraction off), so the instruction sequence is fixed by the source.
+// A wavefront issues one instruction at a time whether one lane or 64 are
+// active, so the sub-step of a lone collapsing bubble costs its instruction
+// count: the polynomial coefficients are read from a constant table into
+// scalar registers right where they are used (not kept live across the
+// sub-step loop, where they would be spilled) and each Horner step is one
+// three-operand v_fma_f64.
 // T = double is the forward.  T = dd (value, tangent) is the adjoint's
 // linearisation: every dd operation computes its value with the same single
 // operation the double code uses, so the value part -- and with it the
@@ -38,6 +44,16 @@ __device__ inline double drcp(double x) {
     r = fma(r, e, r);
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
+}
+
+// Reciprocal square root of a positive normal double: hardware estimate y,
+// e = 1 - x y^2, then y (1 + e/2 + 3 e^2/8) (second-order Newton-Raphson
+// step for x^(-1/2); the estimate's error squared and cubed are far below
+// one ulp).
+__device__ inline double drsqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(y * -x, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 struct dd {
@@ -66,9 +82,9 @@ __device__ inline dd rcp_(dd x) {
     const double r = drcp(x.v);
     return dd(r, -r * r * x.d);
 }
-__device__ inline double rsqrt_(double x) { return rsqrt(x); }
+__device__ inline double rsqrt_(double x) { return drsqrt(x); }
 __device__ inline dd rsqrt_(dd x) {
-    const double r = rsqrt(x.v);
+    const double r = drsqrt(x.v);
     return dd(r, -0.5 * r * r * r * x.d);
 }
 #ifndef BRP_OWN_TRANSCENDENTALS
@@ -76,27 +92,52 @@ __device__ inline dd rsqrt_(dd x) {
 #endif
 
 #if BRP_OWN_TRANSCENDENTALS
+// Polynomial coefficients and reduction constants of expm1_neg and
+// sincos_small: [0, 12) 1/14!, 1/13!, ..., 1/3! (e^r - 1); [12, 20) the sine
+// series -1/17!, 1/15!, ..., 1/3!; [20, 28) the cosine series 1/18!, -1/16!,
+// ..., -1/4!.
+typedef __attribute__((address_space(4))) const double cdouble;
+static __constant__ double kRpPoly[28] = {
+    1.0 / 87178291200.0,  1.0 / 6227020800.0,     1.0 / 479001600.0,
+    1.0 / 39916800.0,     1.0 / 3628800.0,        1.0 / 362880.0,
+    1.0 / 40320.0,        1.0 / 5040.0,           1.0 / 720.0,
+    1.0 / 120.0,          1.0 / 24.0,             1.0 / 6.0,
+    -1.0 / 355687428096000.0, 1.0 / 1307674368000.0, -1.0 / 6227020800.0,
+    1.0 / 39916800.0,     -1.0 / 362880.0,        1.0 / 5040.0,
+    -1.0 / 120.0,         1.0 / 6.0,
+    1.0 / 6402373705728000.0, -1.0 / 20922789888000.0, 1.0 / 87178291200.0,
+    -1.0 / 479001600.0,   1.0 / 3628800.0,        -1.0 / 40320.0,
+    1.0 / 720.0,          -1.0 / 24.0};
+
+// The table's address, opaque to the optimiser so that its loads are not
+// hoisted out of the sub-step loop.
+__device__ inline cdouble *rp_poly() {
+    cdouble *c = (cdouble *)kRpPoly;
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
+// a b + c with c in scalar registers, as one v_fma_f64 (the compiler would
+// otherwise copy c into the destination for a two-operand v_fmac_f64).
+__device__ inline double fma_sc(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
 // exp(x) - 1 for -40 <= x <= -0.14 (the range alpha_coth_m1_g uses):
 // x = n ln2 + r, |r| <= ln2 / 2 (Cody-Waite reduction, ln2 in two parts),
 // e^r - 1 by its Taylor series to r^14 (truncation < 2e-19), then
 // 2^n (1 + p) - 1, which for these x is in [-1, -0.13] (no cancellation
 // beyond one ulp of 1).
 __device__ inline double expm1_neg(double x) {
+    cdouble *c = rp_poly();
     const double n = rint(x * 1.4426950408889634);
     double r = fma(-n, 6.93147180369123816490e-01, x);
     r = fma(-n, 1.90821492927058770002e-10, r);
-    double p = 1.0 / 87178291200.0;  // 1/14!
-    p = fma(p, r, 1.0 / 6227020800.0);
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
+    double p = c[0];
+#pragma unroll
+    for (int k = 1; k < 12; k++) p = fma_sc(p, r, c[k]);
     p = fma(p, r, 0.5);
     p = fma(p * r, r, r);  // e^r - 1
     return n == 0.0 ? p : fma(ldexp(1.0, (int)n), p, ldexp(1.0, (int)n) - 1.0);
@@ -107,27 +148,18 @@ __device__ inline double expm1_neg(double x) {
 // series to r^17 (sin) and r^18 (cos), truncation < 1e-19, then the
 // quadrant of k.
 __device__ inline void sincos_small(double x, double *s, double *c) {
+    cdouble *t = rp_poly();
     const double k = rint(x * 0.63661977236758134308);
     double r = fma(-k, 1.57079632679489655800e+00, x);
     r = fma(-k, 6.12323399573676603587e-17, r);
     const double r2 = r * r;
-    double ps = -1.0 / 355687428096000.0;  // -1/17!
-    ps = fma(ps, r2, 1.0 / 1307674368000.0);
-    ps = fma(ps, r2, -1.0 / 6227020800.0);
-    ps = fma(ps, r2, 1.0 / 39916800.0);
-    ps = fma(ps, r2, -1.0 / 362880.0);
-    ps = fma(ps, r2, 1.0 / 5040.0);
-    ps = fma(ps, r2, -1.0 / 120.0);
-    ps = fma(ps, r2, 1.0 / 6.0);
+    double ps = t[12];
+#pragma unroll
+    for (int q = 13; q < 20; q++) ps = fma_sc(ps, r2, t[q]);
     ps = fma(-(ps * r2), r, r);  // r - r^3/6 + ...
-    double pc = 1.0 / 6402373705728000.0;  // 1/18!
-    pc = fma(pc, r2, -1.0 / 20922789888000.0);
-    pc = fma(pc, r2, 1.0 / 87178291200.0);
-    pc = fma(pc, r2, -1.0 / 479001600.0);
-    pc = fma(pc, r2, 1.0 / 3628800.0);
-    pc = fma(pc, r2, -1.0 / 40320.0);
-    pc = fma(pc, r2, 1.0 / 720.0);
-    pc = fma(pc, r2, -1.0 / 24.0);
+    double pc = t[20];
+#pragma unroll
+    for (int q = 21; q < 28; q++) pc = fma_sc(pc, r2, t[q]);
     pc = fma(pc, r2, 0.5);
     pc = fma(-pc, r2, 1.0);  // 1 - r^2/2 + ...
     const int q = ((int)k) & 3;
@@ -266,7 +298,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     double dtp = fmin(1.01 * dtm, s.dt0);
     const double ad = fabs(val(d1R)), aa = fabs(val(Rnd2R));
     if (ad > kDblEps) dtp = fmin(dtp, (0.01 * val(Rn)) * drcp(ad));
-    if (aa > kDblEps) dtp = fmin(dtp, (0.1 * val(Rn)) * rsqrt(aa));
+    if (aa > kDblEps) dtp = fmin(dtp, (0.1 * val(Rn)) * drsqrt(aa));
     const double idtm = drcp(dtm);
     const double r = dtp * idtm;
     const T acc = (dtp * Rnd2R) * iRn;  // (dtp / Rn) Rnd2R
