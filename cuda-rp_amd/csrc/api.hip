@@ -1258,6 +1258,15 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
          (rc = dalloc(h, (void **)&h->heat_d, sizeof(double) * h->frame))))
         return fail(rc);
     if ((rc = dalloc(h, (void **)&h->sc_d, sizeof(StepScalars)))) return fail(rc);
+    {
+        double v[16] = {};
+        rp_fast_shared_values(bp, v);
+        double *tab = nullptr;
+        if ((rc = dalloc(h, (void **)&tab, sizeof v))) return fail(rc);
+        if (hipMemcpy(tab, v, sizeof v, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(h, BRP_ERR_DEVICE, "hipMemcpy failed"));
+        bp.rp_shared = tab;
+    }
     if (hipHostMalloc((void **)&h->sc_h, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipHostMalloc failed"));
     // init_bub_array (bubble_CUDA.cu:797-897): one bubble per cell centre in the box

@@ -96,7 +96,30 @@ struct BubbleParams {
     int rp_exact;
     int heavy_threshold;    // sub-steps that put a bubble at the queue front
     int extreme_threshold;  // sub-steps that give a bubble a wavefront of its own
+    const double *rp_shared;  // device copy of rp_fast_shared_values (15 doubles)
 };
+
+// The constants shared by every bubble in the shortened-chain RP sub-step, in
+// the field order of RPFastShared (rp_device.h): PL0, gam, mu, rho, 1/rho,
+// dt0, 3 gam, 3 (gam - 1), 3 (gam - 1) K0 T0, 2 sig, 4 mu, mu^2,
+// -2 sig rho, 4 pi K0, 16 pi mu.
+__host__ __device__ inline void rp_fast_shared_values(const BubbleParams &bp, double v[15]) {
+    v[0] = bp.PL0;
+    v[1] = bp.gam;
+    v[2] = bp.mu;
+    v[3] = bp.rho;
+    v[4] = 1.0 / bp.rho;
+    v[5] = bp.dt0;
+    v[6] = 3.0 * bp.gam;
+    v[7] = 3.0 * (bp.gam - 1.0);
+    v[8] = 3.0 * (bp.gam - 1.0) * bp.K0 * bp.T0;
+    v[9] = 2.0 * bp.sig;
+    v[10] = 4.0 * bp.mu;
+    v[11] = bp.mu * bp.mu;
+    v[12] = -2.0 * bp.sig * bp.rho;
+    v[13] = 4.0 * bp.pi * bp.K0;
+    v[14] = 16.0 * bp.pi * bp.mu;
+}
 
 }  // namespace brp
 

@@ -493,21 +493,7 @@ __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, B
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if ((gid >> 3) >= b.n) return;
     const int idx = order ? order[gid >> 3] : gid >> 3, d = gid & 7;
-    const RPFastShared fs{bp.PL0,
-                          bp.gam,
-                          bp.mu,
-                          bp.rho,
-                          1.0 / bp.rho,
-                          bp.dt0,
-                          3.0 * bp.gam,
-                          3.0 * (bp.gam - 1.0),
-                          3.0 * (bp.gam - 1.0) * bp.K0 * bp.T0,
-                          2.0 * bp.sig,
-                          4.0 * bp.mu,
-                          bp.mu * bp.mu,
-                          -2.0 * bp.sig * bp.rho,
-                          4.0 * bp.pi * bp.K0,
-                          16.0 * bp.pi * bp.mu};
+    const RPFastShared fs = rp_shared_load(bp.rp_shared);
     const double R0 = b.R0[idx];
     const double PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
     const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,

@@ -268,21 +268,6 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
     const RPShared sh{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
-    const RPFastShared fs{bp.PL0,
-                          bp.gam,
-                          bp.mu,
-                          bp.rho,
-                          1.0 / bp.rho,
-                          bp.dt0,
-                          3.0 * bp.gam,
-                          3.0 * (bp.gam - 1.0),
-                          3.0 * (bp.gam - 1.0) * bp.K0 * bp.T0,
-                          2.0 * bp.sig,
-                          4.0 * bp.mu,
-                          bp.mu * bp.mu,
-                          -2.0 * bp.sig * bp.rho,
-                          4.0 * bp.pi * bp.K0,
-                          16.0 * bp.pi * bp.mu};
     RPFastConst fk{0, 0, 0, 0, 0, 0, 0};
     // The E longest bubbles (the queue's first entries) run one per workgroup
     // in lane 0 of its first wavefront, alone: a wavefront issues the same
@@ -368,19 +353,31 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
         }
         const bool active = idx >= 0;
         if (__ballot(active) == 0ull) break;
-        // ---- one sub-step, or write back a bubble that reached the sync time
-        if (active) {
-            const int counter = FAST ? LF.counter : L.counter;
-            const bool capped = counter >= bp.max_substeps;
-            double remain = FAST ? LF.remain : L.remain;
-            if (remain > 0.0 && !capped) {
-                if (FAST)
-                    rp_substep_g(LF, fk, fs);
-                else
-                    rp_substep(L, k, sh);
+        // ---- sub-steps until some lane's bubble reaches the sync time (or
+        // the sub-step cap); the refill and write-back code stays outside
+        // this loop, and with it the values only that code needs
+        bool fin, capped;
+        double remain;
+        while (true) {
+            capped = false;
+            remain = 0.0;
+            if (active) {
+                capped = (FAST ? LF.counter : L.counter) >= bp.max_substeps;
                 remain = FAST ? LF.remain : L.remain;
+                if (remain > 0.0 && !capped) {
+                    if (FAST)
+                        rp_substep_g(LF, fk, rp_shared_load(bp.rp_shared));
+                    else
+                        rp_substep(L, k, sh);
+                    remain = FAST ? LF.remain : L.remain;
+                }
             }
-            if (!(remain > 0.0) || capped) {
+            fin = active && (!(remain > 0.0) || capped);
+            if (__ballot(fin) != 0ull) break;
+        }
+        // ---- write back the bubbles that reached the sync time
+        if (active) {
+            if (fin) {
                 RPState out;
                 double R_t, Q_B;
                 if (FAST)

@@ -87,6 +87,19 @@ __device__ inline dd rsqrt_(dd x) {
     const double r = drsqrt(x.v);
     return dd(r, -0.5 * r * r * r * x.d);
 }
+// Constant (scalar-loadable) memory.
+typedef __attribute__((address_space(4))) const double cdouble;
+
+// The shared constants from their device table (rp_fast_shared_values),
+// read where the sub-step uses them rather than held in scalar registers
+// across the sub-step loop.
+__device__ inline RPFastShared rp_shared_load(const double *tab) {
+    cdouble *t = (cdouble *)tab;
+    asm volatile("" : "+s"(t));
+    return RPFastShared{t[0], t[1], t[2],  t[3],  t[4],  t[5],  t[6], t[7],
+                        t[8], t[9], t[10], t[11], t[12], t[13], t[14]};
+}
+
 #ifndef BRP_OWN_TRANSCENDENTALS
 #define BRP_OWN_TRANSCENDENTALS 1
 #endif
@@ -96,7 +109,6 @@ __device__ inline dd rsqrt_(dd x) {
 // sincos_small: [0, 12) 1/14!, 1/13!, ..., 1/3! (e^r - 1); [12, 20) the sine
 // series -1/17!, 1/15!, ..., 1/3!; [20, 28) the cosine series 1/18!, -1/16!,
 // ..., -1/4!.
-typedef __attribute__((address_space(4))) const double cdouble;
 static __constant__ double kRpPoly[28] = {
     1.0 / 87178291200.0,  1.0 / 6227020800.0,     1.0 / 479001600.0,
     1.0 / 39916800.0,     1.0 / 3628800.0,        1.0 / 362880.0,
