@@ -152,8 +152,8 @@ void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
 template <typename F>
 void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                         const BubbleArrays &b, int cur, int plm, int pln, int plp,
-                        const F *p0, int64_t *fg_target, StepScalars *sc, int slot);
-int bubble_step_blocks(int n_bubbles);
+                        const F *p0, int64_t *fg_target, StepScalars *sc, int slot, bool lean);
+int bubble_step_blocks(int n_bubbles, bool lean);
 void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
