@@ -141,6 +141,7 @@ struct brp_handle {
 
     StepScalars *sc_d = nullptr;
     StepScalars *sc_h = nullptr;
+    RPLaunch *rpl_h = nullptr, *rpl_d = nullptr;  // the RP kernel's fixed parameters
     Window W{1, 0, 1, 0};
     int first_step = 1;
     int64_t nstep = 0;
@@ -633,6 +634,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
     const bool bub = h->cfg.bub_enable && h->nb > 0;
     hipStream_t s = h->stream;
     const size_t hdr = offsetof(StepScalars, resid);
+    if (bub) {  // the RP kernel's parameters as of this call (no step is in flight)
+        *h->rpl_h = RPLaunch{h->g, h->bp, h->b};
+        HIPCHK(h, hipMemcpyAsync(h->rpl_d, h->rpl_h, sizeof(RPLaunch), hipMemcpyHostToDevice, s));
+    }
     F *vx = (F *)h->vx, *vy = (F *)h->vy, *p0 = (F *)h->p0;
     const F *cxa = (const F *)h->cxa, *cxb = (const F *)h->cxb, *cya = (const F *)h->cya,
             *cyb = (const F *)h->cyb, *ax = (const F *)h->ax, *bx = (const F *)h->bx,
@@ -727,7 +732,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 if (tin)  // replay: the void fraction from the taped radii, no RP
                     launch_scatter_radii(s, h->g, h->bp, h->b, tin->Rt_it[loop], h->fgb[target], h->sc_d);
                 else
-                    launch_bubble_step<F>(s, h->g, h->bp, h->b, h->rp_cur, h->pl_m, h->pl_n,
+                    launch_bubble_step<F>(s, *h->rpl_h, h->rpl_d, h->rp_cur, h->pl_m, h->pl_n,
                                           h->pl_p, p0, h->fgb[target], h->sc_d, loop,
                                           h->last_sub_max >= 2 * h->bp.extreme_threshold);
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
@@ -1270,8 +1275,10 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
             return fail(set_err(h, BRP_ERR_DEVICE, "hipMemcpy failed"));
         bp.rp_shared = tab;
     }
-    if (hipHostMalloc((void **)&h->sc_h, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&h->sc_h, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&h->rpl_h, sizeof(RPLaunch), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipHostMalloc failed"));
+    if ((rc = dalloc(h, (void **)&h->rpl_d, sizeof(RPLaunch)))) return fail(rc);
     // init_bub_array (bubble_CUDA.cu:797-897): one bubble per cell centre in the box
     if (c.bub_enable) {
         for (int i = 1; i <= h->N; i++) {
@@ -1301,6 +1308,7 @@ void brp_destroy(brp_handle *h) {
         for (void *p : sn.bufs) (void)hipFree(p);
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->sc_h) (void)hipHostFree(h->sc_h);
+    if (h->rpl_h) (void)hipHostFree(h->rpl_h);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);

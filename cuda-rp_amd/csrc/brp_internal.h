@@ -99,6 +99,15 @@ struct BubbleParams {
     const double *rp_shared;  // device copy of rp_fast_shared_values (15 doubles)
 };
 
+// The persistent RP kernel's fixed parameters, in device memory: the kernel
+// reads them where its refill and write-back code uses them instead of
+// holding them in scalar registers across its sub-step loop.
+struct RPLaunch {
+    GridParams g;
+    BubbleParams bp;
+    BubbleArrays b;
+};
+
 // The constants shared by every bubble in the shortened-chain RP sub-step, in
 // the field order of RPFastShared (rp_device.h): PL0, gam, mu, rho, 1/rho,
 // dt0, 3 gam, 3 (gam - 1), 3 (gam - 1) K0 T0, 2 sig, 4 mu, mu^2,
@@ -150,9 +159,9 @@ template <typename F>
 void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, const F *vx, const F *vy, StepScalars *sc);
 template <typename F>
-void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
-                        const BubbleArrays &b, int cur, int plm, int pln, int plp,
-                        const F *p0, int64_t *fg_target, StepScalars *sc, int slot, bool lean);
+void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev, int cur,
+                        int plm, int pln, int plp, const F *p0, int64_t *fg_target,
+                        StepScalars *sc, int slot, bool lean);
 int bubble_step_blocks(int n_bubbles, bool lean);
 void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,

@@ -258,16 +258,34 @@ __device__ inline unsigned long long lanemask_lt(int lane) {
 // per Picard iteration, so a wavefront costs the sum, not the max, of its
 // lanes' sub-steps.  Per bubble the arithmetic is exactly that of the
 // reference kernel, so results do not depend on the schedule.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const RPLaunch crpl;  // scalar-loadable
+#else
+typedef const RPLaunch crpl;  // the host pass only parses kernel bodies
+#endif
+// The launch parameters, through a pointer opaque to the optimiser so that
+// each use site loads what it needs (scalar loads) rather than the kernel
+// keeping every field live across the sub-step loop.
+__device__ inline crpl *rpl(const RPLaunch *p) {
+    crpl *q = (crpl *)p;
+    asm volatile("" : "+s"(q));
+    return q;
+}
+
 template <typename F, bool FAST>
-__global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleParams bp,
-                                                            BubbleArrays b, int cur, int plm,
-                                                            int pln, int plp,
+__global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, int max_substeps,
+                                                            const double *rp_shared, int cur,
+                                                            int plm, int pln, int plp,
                                                             const F *__restrict__ p0,
                                                             int64_t *__restrict__ fg_target,
                                                             StepScalars *sc, int slot) {
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
-    const RPShared sh{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
+    RPShared sh{};
+    if (!FAST) {
+        const BubbleParams &bp = rpl(kp)->bp;
+        sh = RPShared{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
+    }
     RPFastConst fk{0, 0, 0, 0, 0, 0, 0};
     // The E longest bubbles (the queue's first entries) run one per workgroup
     // in lane 0 of its first wavefront, alone: a wavefront issues the same
@@ -277,11 +295,16 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
     // other lanes' refills.  The wavefront runs at raised issue priority
     // until that bubble is done, then joins the shared queue, which starts
     // after the E dedicated entries.
-    const int E = min(b.part[(b.n + 255) / 256], (int)gridDim.x);
+    int E, nb;
+    {
+        const BubbleArrays &b = rpl(kp)->b;
+        nb = b.n;
+        E = min(b.part[(b.n + 255) / 256], (int)gridDim.x);
+    }
     int idx = -1;                  // bubble of this lane, -1 when idle
     bool pre = false;
     if (threadIdx.x == 0 && (int)blockIdx.x < E) {
-        idx = b.order[blockIdx.x];
+        idx = rpl(kp)->b.order[blockIdx.x];
         pre = true;
     }
     bool solo = threadIdx.x < 64 && (int)blockIdx.x < E;  // wave-uniform
@@ -303,6 +326,11 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
         }
         // ---- refill idle lanes from the wave's batch / the global queue
         bool need = idx < 0 && !solo;
+        {
+        crpl *P = rpl(kp);
+        const GridParams &g = P->g;
+        const BubbleParams &bp = P->bp;
+        const BubbleArrays &b = P->b;
         unsigned long long m = __ballot(need);
         bool fresh = pre;
         pre = false;
@@ -312,12 +340,12 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
                 unsigned int base = 0;
                 if (lane == leader) base = atomicAdd(&sc->work[slot], 64u);
                 base = __shfl(base, leader, 64);
-                if ((int)base >= b.n - E) {
+                if ((int)base >= nb - E) {
                     exhausted = true;
                     break;
                 }
                 batch_next = (int)base + E;
-                batch_end = min((int)base + E + 64, b.n);
+                batch_end = min((int)base + E + 64, nb);
             }
             const int avail = batch_end - batch_next;
             const int rank = __popcll(m & lanemask_lt(lane));
@@ -351,6 +379,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
                 rp_begin(L, in, b.PL[plm][idx], b.PL[pln][idx], PL, g.dt, sh);
             }
         }
+        }
         const bool active = idx >= 0;
         if (__ballot(active) == 0ull) break;
         // ---- sub-steps until some lane's bubble reaches the sync time (or
@@ -362,11 +391,11 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
             capped = false;
             remain = 0.0;
             if (active) {
-                capped = (FAST ? LF.counter : L.counter) >= bp.max_substeps;
+                capped = (FAST ? LF.counter : L.counter) >= max_substeps;
                 remain = FAST ? LF.remain : L.remain;
                 if (remain > 0.0 && !capped) {
                     if (FAST)
-                        rp_substep_g(LF, fk, rp_shared_load(bp.rp_shared));
+                        rp_substep_g(LF, fk, rp_shared_load(rp_shared));
                     else
                         rp_substep(L, k, sh);
                     remain = FAST ? LF.remain : L.remain;
@@ -377,6 +406,10 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(GridParams g, BubbleP
         }
         // ---- write back the bubbles that reached the sync time
         if (active) {
+            crpl *P = rpl(kp);
+            const GridParams &g = P->g;
+            const BubbleParams &bp = P->bp;
+            const BubbleArrays &b = P->b;
             if (fin) {
                 RPState out;
                 double R_t, Q_B;
@@ -513,17 +546,18 @@ int bubble_step_blocks(int n_bubbles, bool lean) {
 }
 
 template <typename F>
-void launch_bubble_step(hipStream_t s, const GridParams &g, const BubbleParams &bp,
-                        const BubbleArrays &b, int cur, int plm, int pln, int plp, const F *p0,
-                        int64_t *fg_target, StepScalars *sc, int slot, bool lean) {
-    if (b.n <= 0) return;
-    const dim3 grid(bubble_step_blocks(b.n, lean));
-    if (bp.rp_exact)
-        hipLaunchKernelGGL((k_bubble_persistent<F, false>), grid, dim3(256), 0, s, g, bp, b, cur,
-                           plm, pln, plp, p0, fg_target, sc, slot);
+void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev, int cur,
+                        int plm, int pln, int plp, const F *p0, int64_t *fg_target,
+                        StepScalars *sc, int slot, bool lean) {
+    if (host.b.n <= 0) return;
+    const dim3 grid(bubble_step_blocks(host.b.n, lean));
+    const int cap = host.bp.max_substeps;
+    if (host.bp.rp_exact)
+        hipLaunchKernelGGL((k_bubble_persistent<F, false>), grid, dim3(256), 0, s, dev, cap,
+                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot);
     else
-        hipLaunchKernelGGL((k_bubble_persistent<F, true>), grid, dim3(256), 0, s, g, bp, b, cur,
-                           plm, pln, plp, p0, fg_target, sc, slot);
+        hipLaunchKernelGGL((k_bubble_persistent<F, true>), grid, dim3(256), 0, s, dev, cap,
+                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot);
 }
 
 // Work-queue order (longest-first scheduling: long sequential sub-step
@@ -708,11 +742,11 @@ template void launch_motion<double>(hipStream_t, const GridParams &, const Bubbl
 template void launch_motion<float>(hipStream_t, const GridParams &, const BubbleParams &,
                                    const BubbleArrays &, const float *, const float *,
                                    StepScalars *);
-template void launch_bubble_step<double>(hipStream_t, const GridParams &, const BubbleParams &,
-                                         const BubbleArrays &, int, int, int, int,
-                                         const double *, int64_t *, StepScalars *, int, bool);
-template void launch_bubble_step<float>(hipStream_t, const GridParams &, const BubbleParams &,
-                                        const BubbleArrays &, int, int, int, int, const float *,
-                                        int64_t *, StepScalars *, int, bool);
+template void launch_bubble_step<double>(hipStream_t, const RPLaunch &, const RPLaunch *, int, int,
+                                         int, int, const double *, int64_t *, StepScalars *,
+                                         int, bool);
+template void launch_bubble_step<float>(hipStream_t, const RPLaunch &, const RPLaunch *, int, int,
+                                         int, int, const float *, int64_t *, StepScalars *,
+                                         int, bool);
 
 }  // namespace brp

@@ -201,11 +201,17 @@ struct cx {
     T re, im;
 };
 
-// alpha coth(alpha) - 1 for alpha = s (1 + i), s > 0.  The callers use it
-// for s > 0.1 / sqrt(2) only, so expm1_ sees -40 <= -2s < -0.14.
+// alpha coth(alpha) - 1 = (re + i im) / d for alpha = s (1 + i), s > 0,
+// with the division left to the caller, which folds it into its own
+// reciprocal (one reciprocal per caller instead of two).  The callers use
+// it for s > 0.1 / sqrt(2) only, so expm1_ sees -40 <= -2s < -0.14.
 template <typename T>
-__device__ inline cx<T> alpha_coth_m1_g(T s) {
-    if (val(s) > 20.0) return cx<T>{s - T(1.0), s};
+struct cxd {
+    T re, im, d;
+};
+template <typename T>
+__device__ inline cxd<T> alpha_coth_m1_g(T s) {
+    if (val(s) > 20.0) return cxd<T>{s - T(1.0), s, T(1.0)};
     const T m = expm1_(-2.0 * s);  // exp(-2s) - 1
     T sn, cn;
     sincos_(s, &sn, &cn);
@@ -214,40 +220,47 @@ __device__ inline cx<T> alpha_coth_m1_g(T s) {
     const T c2 = fma_(-2.0 * sn, sn, T(1.0));  // cos 2s
     const T ei = -(e2 * s2);                   // Im E
     const T one_m_er = fma_(2.0 * sn, sn, -(m * c2));  // 1 - Re E
-    const T iD = rcp_(fma_(one_m_er, one_m_er, ei * ei));
-    const T cr = -(m * (m + T(2.0))) * iD;     // (1 - |E|^2) / |1 - E|^2
-    const T ci = (2.0 * ei) * iD;
-    return cx<T>{fma_(s, cr - ci, T(-1.0)), s * (cr + ci)};
+    const T D = fma_(one_m_er, one_m_er, ei * ei);     // |1 - E|^2
+    const T nr = -(m * (m + T(2.0)));                  // 1 - |E|^2
+    const T ni = 2.0 * ei;
+    // coth(alpha) = (nr + i ni) / D
+    return cxd<T>{fma_(s, nr - ni, -D), s * (nr + ni), D};
 }
 
-// Upsilon(alpha) (bubble_CUDA_kernel.cuh:296-311) from s and 1/s.
+// Upsilon(alpha) (bubble_CUDA_kernel.cuh:296-311) from s and 1/s:
+// 3 gam / (1 + k3 (alpha coth alpha - 1) / alpha^2) with 1/alpha^2 =
+// -i / (2 s^2), written as 3 gam D (A - i B) / (A^2 + B^2).
 template <typename T>
 __device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3) {
-    T tr, ti;  // (alpha coth alpha - 1) / alpha^2
+    T A, B, D;
     if (val(s) * 1.4142135623730951 > 1.0e-1) {
-        const cx<T> u = alpha_coth_m1_g(s);
-        const T h = 0.5 * (is * is);  // 1 / (2 s^2)
-        tr = u.im * h;
-        ti = -(u.re * h);
+        const cxd<T> u = alpha_coth_m1_g(s);
+        const T q = k3 * (0.5 * (is * is));  // k3 / (2 s^2)
+        A = fma_(q, u.im, u.d);
+        B = -(q * u.re);
+        D = u.d;
     } else {  // 1/3 + a^2 (-1/45 + a^2 (2/945 - a^2/4725)),  a^2 = i q
         const T q = 2.0 * (s * s);
         T ur = T(2.0 / 945.0), ui = -(q * (1.0 / 4725.0));
         T nr = fma_(-q, ui, T(-1.0 / 45.0)), ni = q * ur;
-        tr = fma_(-q, ni, T(1.0 / 3.0));
-        ti = q * nr;
+        const T tr = fma_(-q, ni, T(1.0 / 3.0));
+        const T ti = q * nr;
+        A = fma_(T(k3), tr, T(1.0));
+        B = k3 * ti;
+        D = T(1.0);
     }
-    const T dr = fma_(T(k3), tr, T(1.0)), di = k3 * ti;
-    const T f = (3.0 * gam) * rcp_(fma_(dr, dr, di * di));
-    return cx<T>{dr * f, -(di * f)};
+    const T f = ((3.0 * gam) * D) * rcp_(fma_(A, A, B * B));
+    return cx<T>{A * f, -(B * f)};
 }
 
-// solveLp (bubble_CUDA_kernel.cuh:314-320) from s and 1/s.
+// solveLp (bubble_CUDA_kernel.cuh:314-320) from s and 1/s:
+// R (1 / (alpha coth alpha - 1) - 3 / alpha^2).
 template <typename T>
 __device__ inline cx<T> lp_g(T s, T is, T R) {
     if (val(s) * 1.4142135623730951 > 1.0e-1) {
-        const cx<T> u = alpha_coth_m1_g(s);
-        const T iu = rcp_(fma_(u.re, u.re, u.im * u.im));
-        return cx<T>{R * (u.re * iu), R * fma_(1.5 * is, is, -(u.im * iu))};
+        const cxd<T> u = alpha_coth_m1_g(s);
+        const T g = u.d * rcp_(fma_(u.re, u.re, u.im * u.im));
+        return cx<T>{R * (u.re * g), R * fma_(1.5 * is, is, -(u.im * g))};
     }
     // 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875)),  a^2 = i q
     const T q = 2.0 * (s * s);
