@@ -1267,8 +1267,8 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
         return fail(rc);
     if ((rc = dalloc(h, (void **)&h->sc_d, sizeof(StepScalars)))) return fail(rc);
     {
-        double v[16] = {};
-        rp_fast_shared_values(bp, v);
+        double v[kRpTable] = {};
+        rp_table_values(bp, v);
         double *tab = nullptr;
         if ((rc = dalloc(h, (void **)&tab, sizeof v))) return fail(rc);
         if (hipMemcpy(tab, v, sizeof v, hipMemcpyHostToDevice) != hipSuccess)

@@ -97,7 +97,17 @@ __device__ inline RPFastShared rp_shared_load(const double *tab) {
     cdouble *t = (cdouble *)tab;
     asm volatile("" : "+s"(t));
     return RPFastShared{t[0], t[1], t[2],  t[3],  t[4],  t[5],  t[6], t[7],
-                        t[8], t[9], t[10], t[11], t[12], t[13], t[14]};
+                        t[8], t[9], t[10], t[11], t[12], t[13], t[14], tab + 16};
+}
+
+// The series coefficients (kRpPoly), read from the table where each use
+// needs them: the pointer is made opaque per use so that the loads are
+// neither hoisted nor shared across uses (which would keep 56 scalar
+// registers live across the sub-step).
+__device__ inline cdouble *rp_poly(const double *poly) {
+    cdouble *c = (cdouble *)poly;
+    asm volatile("" : "+s"(c));
+    return c;
 }
 
 #ifndef BRP_OWN_TRANSCENDENTALS
@@ -105,30 +115,6 @@ __device__ inline RPFastShared rp_shared_load(const double *tab) {
 #endif
 
 #if BRP_OWN_TRANSCENDENTALS
-// Polynomial coefficients and reduction constants of expm1_neg and
-// sincos_small: [0, 12) 1/14!, 1/13!, ..., 1/3! (e^r - 1); [12, 20) the sine
-// series -1/17!, 1/15!, ..., 1/3!; [20, 28) the cosine series 1/18!, -1/16!,
-// ..., -1/4!.
-static __constant__ double kRpPoly[28] = {
-    1.0 / 87178291200.0,  1.0 / 6227020800.0,     1.0 / 479001600.0,
-    1.0 / 39916800.0,     1.0 / 3628800.0,        1.0 / 362880.0,
-    1.0 / 40320.0,        1.0 / 5040.0,           1.0 / 720.0,
-    1.0 / 120.0,          1.0 / 24.0,             1.0 / 6.0,
-    -1.0 / 355687428096000.0, 1.0 / 1307674368000.0, -1.0 / 6227020800.0,
-    1.0 / 39916800.0,     -1.0 / 362880.0,        1.0 / 5040.0,
-    -1.0 / 120.0,         1.0 / 6.0,
-    1.0 / 6402373705728000.0, -1.0 / 20922789888000.0, 1.0 / 87178291200.0,
-    -1.0 / 479001600.0,   1.0 / 3628800.0,        -1.0 / 40320.0,
-    1.0 / 720.0,          -1.0 / 24.0};
-
-// The table's address, opaque to the optimiser so that its loads are not
-// hoisted out of the sub-step loop.
-__device__ inline cdouble *rp_poly() {
-    cdouble *c = (cdouble *)kRpPoly;
-    asm volatile("" : "+s"(c));
-    return c;
-}
-
 // a b + c with c in scalar registers, as one v_fma_f64 (the compiler would
 // otherwise copy c into the destination for a two-operand v_fmac_f64).
 __device__ inline double fma_sc(double a, double b, double c) {
@@ -142,8 +128,8 @@ __device__ inline double fma_sc(double a, double b, double c) {
 // e^r - 1 by its Taylor series to r^14 (truncation < 2e-19), then
 // 2^n (1 + p) - 1, which for these x is in [-1, -0.13] (no cancellation
 // beyond one ulp of 1).
-__device__ inline double expm1_neg(double x) {
-    cdouble *c = rp_poly();
+__device__ inline double expm1_neg(double x, const double *poly) {
+    cdouble *c = rp_poly(poly);
     const double n = rint(x * 1.4426950408889634);
     double r = fma(-n, 6.93147180369123816490e-01, x);
     r = fma(-n, 1.90821492927058770002e-10, r);
@@ -155,12 +141,14 @@ __device__ inline double expm1_neg(double x) {
     return n == 0.0 ? p : fma(ldexp(1.0, (int)n), p, ldexp(1.0, (int)n) - 1.0);
 }
 
-// sin and cos of 0 <= x <= 64: x = k pi/2 + r, |r| <= pi/4 (Cody-Waite
-// reduction, pi/2 in two parts, each step one rounding through fma), Taylor
-// series to r^17 (sin) and r^18 (cos), truncation < 1e-19, then the
-// quadrant of k.
-__device__ inline void sincos_small(double x, double *s, double *c) {
-    cdouble *t = rp_poly();
+// For 0 <= x <= 64: sa = +-sin x and sc = sin x cos x, the two quantities
+// sin 2x = 2 sc and cos 2x = 1 - 2 sa^2 are made of.  x = k pi/2 + r,
+// |r| <= pi/4 (Cody-Waite reduction, pi/2 in two parts, each step one
+// rounding through fma), Taylor series of sin r to r^17 and cos r to r^18
+// (truncation < 1e-19); sin x = +-(sin r or cos r) by the quadrant k, and
+// sin x cos x = (-1)^k sin r cos r.
+__device__ inline void trig_parts(double x, const double *poly, double *sa, double *sc) {
+    cdouble *t = rp_poly(poly);
     const double k = rint(x * 0.63661977236758134308);
     double r = fma(-k, 1.57079632679489655800e+00, x);
     r = fma(-k, 6.12323399573676603587e-17, r);
@@ -174,26 +162,40 @@ __device__ inline void sincos_small(double x, double *s, double *c) {
     for (int q = 21; q < 28; q++) pc = fma_sc(pc, r2, t[q]);
     pc = fma(pc, r2, 0.5);
     pc = fma(-pc, r2, 1.0);  // 1 - r^2/2 + ...
-    const int q = ((int)k) & 3;
-    const double sa = (q & 1) ? pc : ps, ca = (q & 1) ? ps : pc;
-    *s = (q & 2) ? -sa : sa;
-    *c = ((q + 1) & 2) ? -ca : ca;
+    const bool odd = (((int)k) & 1) != 0;
+    const double p = ps * pc;
+    *sa = odd ? pc : ps;
+    *sc = odd ? -p : p;
 }
-__device__ inline double expm1_(double x) { return expm1_neg(x); }
-__device__ inline void sincos_(double x, double *s, double *c) { sincos_small(x, s, c); }
+__device__ inline double expm1_(double x, const double *poly) { return expm1_neg(x, poly); }
+__device__ inline void trig_parts_(double x, const double *poly, double *sa, double *sc) {
+    trig_parts(x, poly, sa, sc);
+}
 #else
-__device__ inline double expm1_(double x) { return expm1(x); }
-__device__ inline void sincos_(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ inline double expm1_(double x, const double *) { return expm1(x); }
+__device__ inline void trig_parts_(double x, const double *, double *sa, double *sc) {
+    double sn, cn;
+    sincos(x, &sn, &cn);
+    *sa = sn;
+    *sc = sn * cn;
+}
 #endif
-__device__ inline dd expm1_(dd x) {
-    const double m = expm1_(x.v);
+__device__ inline dd expm1_(dd x, const double *poly) {
+    const double m = expm1_(x.v, poly);
     return dd(m, (m + 1.0) * x.d);
 }
-__device__ inline void sincos_(dd x, dd *s, dd *c) {
-    double sn, cn;
-    sincos_(x.v, &sn, &cn);
-    *s = dd(sn, cn * x.d);
-    *c = dd(cn, -sn * x.d);
+// The tangent form: sa = sin x with its tangent cos x dx (the value may
+// differ from the double form's in sign only, which no caller sees: they use
+// sa^2 and sc), sc = sin x cos x with value equal to the double form's.
+__device__ inline void trig_parts_(dd x, const double *poly, dd *sa, dd *sc) {
+    double a, p;
+    trig_parts_(x.v, poly, &a, &p);
+    // sin x and cos x up to a common sign pattern: a = +-sin x, and
+    // p / a = cos x when a = sin x; recover cos x from a and p without a
+    // division by choosing the tangent of sa consistently with its sign
+    const double c = a != 0.0 ? p / a : 1.0;  // cos x * (sign of sa / sign of sin x)
+    *sa = dd(a, c * x.d);
+    *sc = dd(p, (c * c - a * a) * x.d);
 }
 
 template <typename T>
@@ -210,16 +212,16 @@ struct cxd {
     T re, im, d;
 };
 template <typename T>
-__device__ inline cxd<T> alpha_coth_m1_g(T s) {
+__device__ inline cxd<T> alpha_coth_m1_g(T s, const double *poly) {
     if (val(s) > 20.0) return cxd<T>{s - T(1.0), s, T(1.0)};
-    const T m = expm1_(-2.0 * s);  // exp(-2s) - 1
-    T sn, cn;
-    sincos_(s, &sn, &cn);
+    const T m = expm1_(-2.0 * s, poly);  // exp(-2s) - 1
+    T sa, sc;                            // +-sin s, sin s cos s
+    trig_parts_(s, poly, &sa, &sc);
     const T e2 = T(1.0) + m;
-    const T s2 = 2.0 * (sn * cn);              // sin 2s
-    const T c2 = fma_(-2.0 * sn, sn, T(1.0));  // cos 2s
+    const T s2 = 2.0 * sc;                     // sin 2s
+    const T c2 = fma_(-2.0 * sa, sa, T(1.0));  // cos 2s
     const T ei = -(e2 * s2);                   // Im E
-    const T one_m_er = fma_(2.0 * sn, sn, -(m * c2));  // 1 - Re E
+    const T one_m_er = fma_(2.0 * sa, sa, -(m * c2));  // 1 - Re E
     const T D = fma_(one_m_er, one_m_er, ei * ei);     // |1 - E|^2
     const T nr = -(m * (m + T(2.0)));                  // 1 - |E|^2
     const T ni = 2.0 * ei;
@@ -231,10 +233,10 @@ __device__ inline cxd<T> alpha_coth_m1_g(T s) {
 // 3 gam / (1 + k3 (alpha coth alpha - 1) / alpha^2) with 1/alpha^2 =
 // -i / (2 s^2), written as 3 gam D (A - i B) / (A^2 + B^2).
 template <typename T>
-__device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3) {
+__device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3, const double *poly) {
     T A, B, D;
     if (val(s) * 1.4142135623730951 > 1.0e-1) {
-        const cxd<T> u = alpha_coth_m1_g(s);
+        const cxd<T> u = alpha_coth_m1_g(s, poly);
         const T q = k3 * (0.5 * (is * is));  // k3 / (2 s^2)
         A = fma_(q, u.im, u.d);
         B = -(q * u.re);
@@ -256,9 +258,9 @@ __device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3) {
 // solveLp (bubble_CUDA_kernel.cuh:314-320) from s and 1/s:
 // R (1 / (alpha coth alpha - 1) - 3 / alpha^2).
 template <typename T>
-__device__ inline cx<T> lp_g(T s, T is, T R) {
+__device__ inline cx<T> lp_g(T s, T is, T R, const double *poly) {
     if (val(s) * 1.4142135623730951 > 1.0e-1) {
-        const cxd<T> u = alpha_coth_m1_g(s);
+        const cxd<T> u = alpha_coth_m1_g(s, poly);
         const T g = u.d * rcp_(fma_(u.re, u.re, u.im * u.im));
         return cx<T>{R * (u.re * g), R * fma_(1.5 * is, is, -(u.im * g))};
     }
@@ -361,7 +363,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     for (int it = 0; it < 3; it++) {
         const T x = ca * om;
         const T isa = rsqrt_(x);
-        const cx<T> ups = upsilon_g(x * isa, isa, s.gam, s.k3);
+        const cx<T> ups = upsilon_g(x * isa, isa, s.gam, s.k3, s.poly);
         const T ur = ups.re * coef1, ui = ups.im * coef1;
         const T mu_eff = fma_(ui * PGn, 0.25 * iom, T(s.mu));  // mu + ui PG / (4 omega)
         eta = fma_(ur, coef2, fma_(-(mu_eff * mu_eff), coef3, value1));
@@ -369,7 +371,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     }
     const T x = ca * om;
     const T isa = rsqrt_(x);
-    const cx<T> Lp = lp_g(x * isa, isa, Rn);
+    const cx<T> Lp = lp_g(x * isa, isa, Rn, s.poly);
     // solvePG (bubble_CUDA_kernel.cuh:322-340)
     const T Rbar = 0.5 * (Rp + Rn);
     const T iRbar = rcp_(Rbar);
