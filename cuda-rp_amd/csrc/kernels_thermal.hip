@@ -47,37 +47,91 @@ __global__ void k_zero_window(double *q, Window w, int pitch) {
     q[frame_off(i, j, pitch)] = 0.0;
 }
 
-// BubbleHeatKernel (bubble_CUDA_kernel.cuh:972-1006)
+// BubbleHeatKernel (bubble_CUDA_kernel.cuh:972-1006).  Bubbles are
+// cell-sorted, so a workgroup's 256 stencils cover a small patch of cells:
+// the contributions are summed in an LDS tile over the workgroup's bounding
+// box and flushed with one global atomic per touched cell (a patch larger
+// than the tile falls back to global atomics per contribution).  The sums
+// are FP64 in either path; only their order differs.
+constexpr int kHeatTile = 4096;  // cells (32 KB of LDS)
+constexpr int kHeatBand = 8;     // the largest smooth_delta_band (checked at set-up)
 __global__ __launch_bounds__(256) void k_bubble_heat(GridParams g, BubbleParams bp, BubbleArrays b,
                                                       double *heat) {
+    __shared__ double tile[kHeatTile];
+    __shared__ int box[4];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= b.n) return;
-    const double x = b.x[idx], y = b.y[idx];
-    double Q;
-    {
+    const bool valid = idx < b.n;
+    double x = 0.0, y = 0.0, Q = 0.0;
+    int ibnx = 0, ibny = 0;
+    if (valid) {
+        x = b.x[idx];
+        y = b.y[idx];
+        {
 #pragma clang fp contract(off)
-        Q = bp.cylindrical ? b.n_B[idx] * b.Q_B[idx] * g.rdx * g.rdy / x
-                           : b.n_B[idx] * b.Q_B[idx] * g.rdx * g.rdy;
+            Q = bp.cylindrical ? b.n_B[idx] * b.Q_B[idx] * g.rdx * g.rdy / x
+                               : b.n_B[idx] * b.Q_B[idx] * g.rdx * g.rdy;
+        }
+        double v;
+        {
+#pragma clang fp contract(off)
+            v = floor(x * g.rdx + 0.5);
+        }
+        ibnx = (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
+        {
+#pragma clang fp contract(off)
+            v = floor(y * g.rdy + 0.5);
+        }
+        ibny = (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
     }
-    double v;
-    {
-#pragma clang fp contract(off)
-        v = floor(x * g.rdx + 0.5);
+    // this bubble's stencil clipped to the frame's cells [0, N+1] x [0, M+1]
+    const int i0 = max(ibnx + bp.mbs, 0), i1 = min(ibnx + bp.mbe, g.N + 1);
+    const int j0 = max(ibny + bp.mbs, 0), j1 = min(ibny + bp.mbe, g.M + 1);
+    const bool any = valid && i0 <= i1 && j0 <= j1;
+    if (threadIdx.x == 0) {
+        box[0] = 0x7fffffff;
+        box[1] = -1;
+        box[2] = 0x7fffffff;
+        box[3] = -1;
     }
-    const int ibnx = (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
-    {
-#pragma clang fp contract(off)
-        v = floor(y * g.rdy + 0.5);
+    __syncthreads();
+    if (any) {
+        atomicMin(&box[0], i0);
+        atomicMax(&box[1], i1);
+        atomicMin(&box[2], j0);
+        atomicMax(&box[3], j1);
     }
-    const int ibny = (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
-    for (int i = ibnx + bp.mbs; i <= ibnx + bp.mbe; i++) {
-        const double Dx = delta_t(bp.dcx, bp.dcy, i, g.dx, x);
-        for (int j = ibny + bp.mbs; j <= ibny + bp.mbe; j++) {
-            const double Dy = delta_t(bp.dcx, bp.dcz, j, g.dy, y);
-            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+    __syncthreads();
+    const int bx0 = box[0], by0 = box[2];
+    const int bw = box[1] - bx0 + 1, bh = box[3] - by0 + 1;
+    if (bw <= 0 || bh <= 0) return;  // workgroup-uniform: no stencil in the frame
+    const bool binned = (int64_t)bw * bh <= kHeatTile;  // workgroup-uniform
+    if (binned) {
+        for (int k = threadIdx.x; k < bw * bh; k += blockDim.x) tile[k] = 0.0;
+        __syncthreads();
+    }
+    if (any) {
+        double Dy[kHeatBand];
+        for (int j = j0; j <= j1; j++) Dy[j - j0] = delta_t(bp.dcx, bp.dcz, j, g.dy, y);
+        for (int i = i0; i <= i1; i++) {
+            const double Dx = delta_t(bp.dcx, bp.dcy, i, g.dx, x);
+            for (int j = j0; j <= j1; j++) {
+                double c;
+                {
 #pragma clang fp contract(off)
-                atomicAdd(&heat[frame_off(i, j, g.pitch)], Q * Dx * Dy);
+                    c = Q * Dx * Dy[j - j0];
+                }
+                if (binned)
+                    atomicAdd(&tile[(j - by0) * bw + (i - bx0)], c);
+                else
+                    atomicAdd(&heat[frame_off(i, j, g.pitch)], c);
             }
+        }
+    }
+    if (binned) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < bw * bh; k += blockDim.x) {
+            const double v = tile[k];
+            if (v != 0.0) atomicAdd(&heat[frame_off(bx0 + k % bw, by0 + k / bw, g.pitch)], v);
         }
     }
 }
