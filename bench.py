@@ -57,6 +57,26 @@ def c2_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
     return cfg, (x, y, R0, None, fg_in)
 
 
+def c3_workload(abi, precision, n=4096, nb=1 << 22, seed=3):
+    """4096^2, 32-cell PML, 2 MHz 3 MPa focused CW, 2^22 log-normal bubbles
+    (median 3 um, sigma_log 0.4) clustered in 16 Gaussian clouds (centres
+    uniform over an 8 x 8 mm box at the focus, sigma 0.25 mm, clipped to the
+    box), mean void fraction 1e-6 over the box."""
+    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=3.0e6, freq=2.0e6, focal_dist=15e-3,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = np.random.default_rng(seed)
+    fpx, fpy = cfg.ext_focus_x, 15e-3 * 0.5 * math.sqrt(3.0)
+    box = 8e-3
+    cx = fpx + (r.random(16) - 0.5) * box
+    cy = fpy + (r.random(16) - 0.5) * box
+    k = r.integers(0, 16, nb)
+    x = np.clip(cx[k] + 0.25e-3 * r.standard_normal(nb), fpx - 0.5 * box, fpx + 0.5 * box)
+    y = np.clip(cy[k] + 0.25e-3 * r.standard_normal(nb), fpy - 0.5 * box, fpy + 0.5 * box)
+    R0 = np.exp(math.log(3e-6) + 0.4 * r.standard_normal(nb))
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    return cfg, (x, y, R0, None, fg_in)
+
+
 def c4_workload(abi, precision, n=1024, nb=65536, n_elem=64, seed=4):
     """The adjoint workload: 1024^2, 65536 bubbles with R0 ~ U(1, 5) um in a
     4 x 4 mm box at the focus (mean void fraction 1e-6 there), a 64-element
@@ -380,6 +400,13 @@ def main():
             "bubble_steps_per_s": nb * first / wall_c,
             "gpu_same_sample_value": cells * first / tfirst_m,
         }
+        if threads > 1:  # the same oracle on one core, on a shorter sample
+            k1 = max(1, min(2, first))
+            log(f"cpu baseline: oracle, 1 thread, steps 1-{k1}")
+            wall_1, _ = cpu_baseline(abi, cfg_c, bub, k1, 1)
+            out["cpu_baseline"]["single_thread"] = {
+                "value": cells * k1 / wall_1, "unit": "cell-updates/s", "cores": 1,
+                "sample": f"steps 1-{k1} ({wall_1:.1f} s)"}
     print(json.dumps(out), flush=True)
 
 

@@ -1170,6 +1170,9 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
         brp_destroy(h);
         return rc;
     };
+    if (h->frame > (size_t)INT32_MAX)  // the RP scatter keys cells by 32-bit frame offset
+        return fail(set_err(h, BRP_ERR_INVALID, "grid of %d x %d cells exceeds 2^31 frame elements",
+                            h->N, h->M));
     if (hipStreamCreate(&h->stream) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipStreamCreate failed"));
     h->ev.resize(8);

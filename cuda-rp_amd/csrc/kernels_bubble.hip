@@ -239,6 +239,95 @@ __device__ inline void scatter_bubble(const GridParams &g, const BubbleParams &b
     scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp, fg, bad);
 }
 
+// Per-wavefront aggregation of the void-fraction scatter in LDS: an
+// open-addressing table of (frame offset, int64 sum) pairs.  A lane adds its
+// stencil's contributions to the table; the wavefront flushes the table to
+// the grid with one global atomic per distinct cell when it is half full and
+// when the wavefront ends.  Bubbles reach a wavefront in batches of
+// consecutive cell-sorted indices, so its table sees few distinct cells: with
+// clustered bubbles (thousands per cell) this replaces thousands of
+// same-address device atomics by one, and int64 addition keeps the result
+// identical in any order.  A contribution that finds no slot in kAggProbe
+// probes goes to the grid directly.
+constexpr int kAggSlots = 512;  // per wavefront (a power of two)
+constexpr int kAggProbe = 16;
+struct WaveAgg {
+    unsigned long long val[kAggSlots];
+    int key[kAggSlots];  // frame offset, -1 when empty
+    int count;           // occupied slots
+};
+
+__device__ inline void agg_clear(WaveAgg &A, int lane) {
+    for (int k = lane; k < kAggSlots; k += 64) {
+        A.key[k] = -1;
+        A.val[k] = 0ull;
+    }
+    if (lane == 0) A.count = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) unsigned long long lds_u64_t;
+#else
+typedef unsigned long long lds_u64_t;  // the host pass only parses kernel bodies
+#endif
+__device__ inline lds_u64_t *lds_u64(unsigned long long *p) { return (lds_u64_t *)p; }
+
+__device__ inline void agg_add(WaveAgg &A, int cell, unsigned long long q, int64_t *fg) {
+    int h = (int)(((unsigned)cell * 2654435761u) >> 23) & (kAggSlots - 1);
+    for (int probe = 0; probe < kAggProbe; probe++) {
+        const int k = atomicCAS(&A.key[h], -1, cell);
+        if (k == -1) atomicAdd(&A.count, 1);
+        if (k == -1 || k == cell) {
+            __hip_atomic_fetch_add(lds_u64(&A.val[h]), q, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+            return;
+        }
+        h = (h + 1) & (kAggSlots - 1);
+    }
+    atomicAdd((unsigned long long *)&fg[cell], q);
+}
+
+// Whole wavefront: the table's sums to the grid, then an empty table.
+__device__ inline void agg_flush(WaveAgg &A, int lane, int64_t *fg) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int k = lane; k < kAggSlots; k += 64) {
+        const int c = A.key[k];
+        if (c >= 0) {
+            const unsigned long long v = A.val[k];
+            if (v) atomicAdd((unsigned long long *)&fg[c], v);
+        }
+    }
+    agg_clear(A, lane);
+}
+
+// scatter_one with the contributions added to the wavefront's table.
+__device__ inline void scatter_bubble_agg(const GridParams &g, const BubbleParams &bp, double x,
+                                          double y, int ibnx, int ibny, double fg_temp,
+                                          WaveAgg &A, int64_t *fg, bool *bad) {
+    const int nb = bp.mbe - bp.mbs + 1;
+    double wx[kMaxBand], wy[kMaxBand];
+    for (int a = 0; a < nb; a++) {
+        wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
+        wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
+    }
+    for (int a = 0; a < nb; a++) {
+        const int i = ibnx + bp.mbs + a;
+        for (int c = 0; c < nb; c++) {
+            const int j = ibny + bp.mbs + c;
+            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+                double v;
+                {
+#pragma clang fp contract(off)
+                    v = fg_temp * wx[a] * wy[c];
+                }
+                const int64_t q = to_fixed(v, bad);
+                if (q != 0) agg_add(A, (int)frame_off(i, j, g.pitch), (unsigned long long)q, fg);
+            }
+        }
+    }
+}
+
 __device__ inline unsigned long long lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
@@ -281,6 +370,9 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                                                             StepScalars *sc, int slot) {
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
+    __shared__ WaveAgg agg_all[4];  // 25 KB: one table per wavefront
+    WaveAgg &agg = agg_all[threadIdx.x >> 6];
+    agg_clear(agg, lane);
     RPShared sh{};
     if (!FAST) {
         const BubbleParams &bp = rpl(kp)->bp;
@@ -431,12 +523,14 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                 sub_sum += (unsigned long long)cnt;
                 capped_any |= capped && remain > 0.0;
                 bad |= !isfinite(R_t);
-                scatter_bubble(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], R_t, x),
-                               fg_target, &bad);
+                scatter_bubble_agg(g, bp, x, y, ibnx, ibny,
+                                   fg_temp_of(g, bp, b.n_B[idx], R_t, x), agg, fg_target, &bad);
                 idx = -1;
             }
         }
+        if (__builtin_amdgcn_readfirstlane(agg.count) > kAggSlots / 2) agg_flush(agg, lane, fg_target);
     }
+    agg_flush(agg, lane, fg_target);
     for (int off = 32; off > 0; off >>= 1) {
         sub_max = max(sub_max, __shfl_xor(sub_max, off, 64));
         sub_sum += __shfl_xor(sub_sum, off, 64);
