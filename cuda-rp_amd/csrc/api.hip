@@ -115,7 +115,8 @@ struct brp_handle {
     int N = 0, M = 0, pitch = 0, rows = 0;
     size_t frame = 0;  // elements per frame
     double T_inf = 0, P_inf = 0, rho_inf = 0, cs_inf = 0, dt = 0, dt0_rp = 0;
-    int last_sub_max = 0;  // longest RP sub-step chain of the last launch's step (grid choice)
+    int last_sub_max = 0;             // longest RP sub-step chain so far in the last step
+    double last_sub_per_launch = 0.0;  // RP sub-steps per launch in the last step (grid choice)
     double fp_x = 0, fp_y = 0, omega = 0, ctrl_x = 0, ctrl_y = 0, ctrl_dist = 0;
     std::vector<double> sig_mx, sig_my, sig_nx, sig_ny;
 
@@ -734,7 +735,8 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 else
                     launch_bubble_step<F>(s, *h->rpl_h, h->rpl_d, h->rp_cur, h->pl_m, h->pl_n,
                                           h->pl_p, p0, h->fgb[target], h->sc_d, loop,
-                                          h->last_sub_max >= 2 * h->bp.extreme_threshold);
+                                          bubble_step_lean(h->last_sub_max, h->last_sub_per_launch,
+                                                           h->bp.extreme_threshold));
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
                 if (rec && rec->want_grid)
                     HIPCHK(h, hipMemcpyAsync(rec->fg_it[loop], h->fgb[target], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
@@ -760,6 +762,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 std::memcpy(&r, &h->sc_h->resid[loop], 8);
                 resimax = r / h->P_inf;
                 h->last_sub_max = h->sc_h->sub_max;
+                h->last_sub_per_launch = (double)h->sc_h->sub_sum / loop;
                 std::swap(target, prev);
                 float a = 0.f, c = 0.f, o = 0.f;
                 (void)hipEventElapsedTime(&o, h->ev[7], h->ev[4]);

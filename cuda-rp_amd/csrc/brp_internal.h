@@ -182,6 +182,10 @@ void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev
                         int plm, int pln, int plp, const F *p0, int64_t *fg_target,
                         StepScalars *sc, int slot, bool lean);
 int bubble_step_blocks(int n_bubbles, bool lean);
+// Whether the next RP launch is bound by its longest chains (lean grid): the
+// longest chain of the last launch is long and the sub-steps per lane of a
+// one-workgroup-per-CU grid are few against it.
+bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold);
 void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);

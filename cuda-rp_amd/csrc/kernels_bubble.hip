@@ -172,31 +172,8 @@ __global__ __launch_bounds__(256) void k_motion(GridParams g, BubbleParams bp, B
     block_bbox(imin, imax, jmin, jmax, b.bbox_blk);
 }
 
-// The scatter of one bubble's volume (VoidFractionReverseLookupKernel,
-// bubble_CUDA_kernel.cuh:576-619) given its stencil weights.
-__device__ inline void scatter_one(const GridParams &g, const BubbleParams &bp, int ibnx,
-                                   int ibny, const double *wx, const double *wy, double fg_temp,
-                                   int64_t *fg, bool *bad) {
-    const int nb = bp.mbe - bp.mbs + 1;
-    for (int a = 0; a < nb; a++) {
-        const int i = ibnx + bp.mbs + a;
-        for (int c = 0; c < nb; c++) {
-            const int j = ibny + bp.mbs + c;
-            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
-                double v;
-                {
-#pragma clang fp contract(off)
-                    v = fg_temp * wx[a] * wy[c];
-                }
-                const int64_t q = to_fixed(v, bad);
-                if (q != 0)
-                    atomicAdd((unsigned long long *)&fg[frame_off(i, j, g.pitch)],
-                              (unsigned long long)q);
-            }
-        }
-    }
-}
-
+// A bubble's volume per cell area, the quantity the scatter spreads
+// (VoidFractionReverseLookupKernel, bubble_CUDA_kernel.cuh:576-619).
 __device__ inline double fg_temp_of(const GridParams &g, const BubbleParams &bp, double n_B,
                                     double R, double x) {
 #pragma clang fp contract(off)
@@ -225,18 +202,6 @@ __device__ inline double gather_p0(const GridParams &g, const BubbleParams &bp, 
         }
     }
     return PL;
-}
-
-__device__ inline void scatter_bubble(const GridParams &g, const BubbleParams &bp, double x,
-                                      double y, int ibnx, int ibny, double fg_temp, int64_t *fg,
-                                      bool *bad) {
-    const int nb = bp.mbe - bp.mbs + 1;
-    double wx[kMaxBand], wy[kMaxBand];
-    for (int a = 0; a < nb; a++) {
-        wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
-        wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
-    }
-    scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp, fg, bad);
 }
 
 // Per-wavefront aggregation of the void-fraction scatter in LDS: an
@@ -301,7 +266,9 @@ __device__ inline void agg_flush(WaveAgg &A, int lane, int64_t *fg) {
     agg_clear(A, lane);
 }
 
-// scatter_one with the contributions added to the wavefront's table.
+// VoidFractionReverseLookupKernel's per-bubble scatter (bubble_CUDA_kernel.cuh:
+// 576-619): fg_temp times the B x B smooth-delta weights, as int64 fixed
+// point, added to the wavefront's table.
 __device__ inline void scatter_bubble_agg(const GridParams &g, const BubbleParams &bp, double x,
                                           double y, int ibnx, int ibny, double fg_temp,
                                           WaveAgg &A, int64_t *fg, bool *bad) {
@@ -549,20 +516,19 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
 __global__ __launch_bounds__(256) void k_scatter_initial(GridParams g, BubbleParams bp,
                                                           BubbleArrays b, int64_t *fg,
                                                           StepScalars *sc) {
+    __shared__ WaveAgg agg_all[4];
+    WaveAgg &agg = agg_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    agg_clear(agg, lane);
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     bool bad = false;
     if (idx < b.n) {
         const double x = b.x[idx], y = b.y[idx];
         const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
-        const int nb = bp.mbe - bp.mbs + 1;
-        double wx[kMaxBand], wy[kMaxBand];
-        for (int a = 0; a < nb; a++) {
-            wx[a] = delta_w(bp.dcx, bp.dcy, ibnx + bp.mbs + a, g.dx, x);
-            wy[a] = delta_w(bp.dcx, bp.dcz, ibny + bp.mbs + a, g.dy, y);
-        }
-        scatter_one(g, bp, ibnx, ibny, wx, wy, fg_temp_of(g, bp, b.n_B[idx], b.R_t[idx], x), fg,
-                    &bad);
+        scatter_bubble_agg(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], b.R_t[idx], x),
+                           agg, fg, &bad);
     }
+    agg_flush(agg, lane, fg);
     if (bad) atomicOr(&sc->flags, 2);
 }
 
@@ -571,14 +537,19 @@ __global__ __launch_bounds__(256) void k_scatter_initial(GridParams g, BubblePar
 __global__ __launch_bounds__(256) void k_scatter_radii(GridParams g, BubbleParams bp,
                                                         BubbleArrays b, const double *Rt,
                                                         int64_t *fg, StepScalars *sc) {
+    __shared__ WaveAgg agg_all[4];
+    WaveAgg &agg = agg_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    agg_clear(agg, lane);
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     bool bad = false;
     if (idx < b.n) {
         const double x = b.x[idx], y = b.y[idx];
         const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
-        scatter_bubble(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], Rt[idx], x), fg,
-                       &bad);
+        scatter_bubble_agg(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], Rt[idx], x),
+                           agg, fg, &bad);
     }
+    agg_flush(agg, lane, fg);
     if (bad) atomicOr(&sc->flags, 2);
 }
 
@@ -621,22 +592,34 @@ void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
 // Workgroups of the persistent RP kernel: as many as fit (two per CU) when
 // the launch is throughput-bound; one per CU ("lean") when a few long
 // sub-step chains bound it, so that each dedicated wavefront has a SIMD of
-// its own and fewer shared-queue wavefronts compete with it (measured at
-// 2048^2 with 2^20 bubbles during cavitation: 12% less RP time; before
-// cavitation, where the launch is throughput-bound, the full grid is 25%
-// faster).
-int bubble_step_blocks(int n_bubbles, bool lean) {
-    static int cus = 0, per_cu = 0;
-    if (!cus) {
+// its own and fewer shared-queue wavefronts compete with it.  Measured with
+// the void-fraction scatter aggregated in LDS: 2048^2, 2^20 bubbles during
+// cavitation (a 3500-sub-step chain, 430 sub-steps per lean-grid lane):
+// lean 23.0 vs full 26.2 ms of RP per step; 4096^2, 2^22 clustered bubbles
+// at 3 MPa (a 4000-sub-step chain, several thousand per lane): lean 84 vs
+// full 60 ms.  bubble_step_lean decides from the previous launch.
+static int g_cus = 0, g_per_cu = 0;
+static void rp_grid_init() {
+    if (!g_cus) {
         int dev = 0, c = 0, p = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_bubble_persistent<float, true>,
                                                            256, 0);
-        per_cu = std::max(1, p);
-        cus = std::max(1, c);
+        g_per_cu = std::max(1, p);
+        g_cus = std::max(1, c);
     }
-    return std::max(1, std::min(cus * (lean ? 1 : per_cu), (n_bubbles + 255) / 256));
+}
+
+bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold) {
+    rp_grid_init();
+    const double lanes = 256.0 * g_cus;  // lanes of the lean grid
+    return sub_max >= 2 * extreme_threshold && sub_per_launch < 0.35 * lanes * sub_max;
+}
+
+int bubble_step_blocks(int n_bubbles, bool lean) {
+    rp_grid_init();
+    return std::max(1, std::min(g_cus * (lean ? 1 : g_per_cu), (n_bubbles + 255) / 256));
 }
 
 template <typename F>
