@@ -242,6 +242,27 @@ def thermal_leg(brp, abi, n, nb, bub, warmup, steps, other_without_ms):
                          "bytes_per_cell": per_cell}}
 
 
+def c3_leg(brp, abi, warmup, steps):
+    """The 4096^2 workload with 2^22 clustered bubbles at 2 MHz / 3 MPa
+    (c3_workload), mixed precision, timed over `steps` steps after `warmup`
+    (the wavefront reaches the clouds near step 200)."""
+    cfg, bub = c3_workload(abi, abi.PREC_MIXED)
+    n, nb = cfg.grid_x, len(bub[0])
+    sim, st, wall, ph, _, _ = run_gpu(brp, cfg, bub, warmup, steps, 0)
+    sim.close()
+    launches = sum(s["n_pic"] for s in st)
+    return {"workload": f"{n}x{n} grid, 32-cell PML, 2 MHz 3 MPa focused CW, {nb} log-normal "
+                        "bubbles in 16 Gaussian clouds (sigma 0.25 mm) in an 8x8 mm box, mixed "
+                        "precision",
+            "steps_window": f"{warmup + 1}-{warmup + steps}",
+            "ms_per_step": wall / steps * 1e3, "value": n * n * steps / wall,
+            "unit": "cell-updates/s", "bubble_steps_per_s": nb * steps / wall,
+            "picard_per_step": launches / steps,
+            "rp_launch_ms": ph["bubble_rp"] / max(launches, 1),
+            "rp_max_substeps": max(s["rp_max_substeps"] for s in st),
+            "phase_ms_per_step": {k: v / steps for k, v in ph.items()}}
+
+
 def calibrate_copy(mib=1024, reps=5):
     """A plain device copy of known size (reads and writes `mib` MiB per
     launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale
@@ -271,6 +292,9 @@ def main():
     ap.add_argument("--bubbles", type=int, default=1 << 20)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-thermal", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=100)
+    ap.add_argument("--c3-warmup", type=int, default=300)
     ap.add_argument("--thermal-steps", type=int, default=100)
     ap.add_argument("--adjoint-steps", type=int, default=4000)
     ap.add_argument("--adjoint-window", type=int, default=1000)
@@ -375,6 +399,11 @@ def main():
         log(f"  fp64: {wall_d / args.steps * 1e3:.3f} ms/step; p0 err {err_p0:.2e}")
         sim_d.close()
     sim_m.close()
+
+    if not args.no_c3:
+        log("C3 leg")
+        out["c3"] = c3_leg(brp, abi, args.c3_warmup, args.c3_steps)
+        log(f"  C3: {out['c3']['ms_per_step']:.2f} ms/step")
 
     if not args.no_thermal:
         log("thermal leg")

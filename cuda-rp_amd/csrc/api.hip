@@ -335,7 +335,8 @@ int alloc_bubbles(brp_handle *h, int n) {
         if ((rc = dalloc(h, (void **)a, bytes))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.sub, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.order, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
-    if ((rc = dalloc(h, (void **)&h->b.order_tmp, sizeof(int) * (size_t)std::max(n, 1))))
+    if ((rc = dalloc(h, (void **)&h->b.order_tmp, sizeof(int) * (size_t)std::max(n, 1))) ||
+        (rc = dalloc(h, (void **)&h->b.sort_hist, sizeof(int) * 2048)))
         return rc;
     if ((rc = dalloc(h, (void **)&h->b.part, sizeof(int) * 2 * (size_t)((n + 255) / 256 + 1))))
         return rc;
@@ -882,6 +883,7 @@ struct AdjState {
     double *lRt = nullptr, *lPLp = nullptr, *lPLn = nullptr, *lPLm = nullptr;
     unsigned int *mismatch = nullptr;  // RP replays whose sub-step count differed
     int *order = nullptr, *order_tmp = nullptr, *part = nullptr;  // longest-first RP order
+    int *sort_hist = nullptr;
 };
 
 // Reverse of one replayed step.  On entry `a` holds the adjoint of the state
@@ -923,6 +925,7 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
         bo.sub = brec.sub_it[k];
         bo.order = a.order;
         bo.order_tmp = a.order_tmp;
+        bo.sort_hist = a.sort_hist;
         bo.part = a.part;
         launch_order_partition(s, bo, h->bp.heavy_threshold, h->bp.extreme_threshold);
         launch_adj_rp(s, g, h->bp, bv, 0, 0, 1, brec.PLp_it[k], a.lRt, k == n ? a.LR : none,
@@ -1044,7 +1047,8 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
     for (double **q : bubs)
         if ((rc = mem.get(h, q, nb))) return rc;
     if ((rc = mem.get(h, &a.mismatch, 1)) || (rc = mem.get(h, &a.order, nb)) ||
-        (rc = mem.get(h, &a.order_tmp, nb)) || (rc = mem.get(h, &a.part, 2 * (nb / 256 + 2))))
+        (rc = mem.get(h, &a.order_tmp, nb)) || (rc = mem.get(h, &a.part, 2 * (nb / 256 + 2))) ||
+        (rc = mem.get(h, &a.sort_hist, 2048)))
         return rc;
 
     // reverse sweep: per segment, the steps are recomputed from the segment's

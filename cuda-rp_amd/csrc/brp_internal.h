@@ -81,6 +81,7 @@ struct BubbleArrays {
     int *sub;             // sub-steps of the last RP call, per bubble
     int *order;           // work-queue order: bubbles with many sub-steps first
     int *order_tmp;       // partition scratch: the extreme class before its sort
+    int *sort_hist;       // partition scratch: 2048 bin counts / offsets of that sort
     int *part;            // partition scratch: per-block extreme and heavy offsets + totals
     int *bbox_blk;        // per-block stencil bounding boxes of k_motion (4 ints each)
 };

@@ -644,9 +644,13 @@ void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev
 // (cell-sorted) order.  Passes: per-block class counts, one-block exclusive
 // scans, scatter (the extreme class into a scratch array), then the sort of
 // the extreme class.
+constexpr int kSortBins = 2048;  // bins of the extreme-class sort
+
 __global__ __launch_bounds__(256) void k_part_count(const int *sub, int n, int ext, int thr,
-                                                    int *cntE, int *cntH) {
+                                                    int *cntE, int *cntH, int *sort_hist) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < kSortBins; k += 256) sort_hist[k] = 0;
     const int v = idx < n ? sub[idx] : 0;
     const int ce = __syncthreads_count(idx < n && v >= ext);
     const int ch = __syncthreads_count(idx < n && v >= thr && v < ext);
@@ -731,24 +735,33 @@ __global__ __launch_bounds__(256) void k_part_scatter(const int *sub, int n, int
 
 // The extreme class, longest first: ext_in[0, E) (E = *totE) scattered into
 // order[0, E) by descending sub-step count, in bins of 8 sub-steps (order
-// within a bin unspecified; results do not depend on the queue order).  One
-// workgroup: bin histogram in LDS, exclusive scan, scatter with LDS atomics.
-constexpr int kSortBins = 2048;
+// within a bin unspecified; results do not depend on the queue order).  A
+// counting sort over many workgroups: bin counts (LDS histograms merged with
+// one atomic per non-empty bin and workgroup), one-workgroup exclusive scan,
+// then per 256-element chunk a range reserved per non-empty bin and LDS ranks
+// within it.  hist (kSortBins ints) is cleared by k_part_count.
 __device__ inline int sort_bin(int v) {  // ascending bin = descending count
     return kSortBins - 1 - min(v >> 3, kSortBins - 1);
 }
 
-__global__ __launch_bounds__(1024) void k_sort_extreme(const int *sub, const int *totE,
-                                                       const int *ext_in, int *order) {
-    __shared__ int hist[kSortBins];
-    __shared__ int wsum[16];
+__global__ __launch_bounds__(256) void k_sort_hist(const int *sub, const int *totE,
+                                                   const int *ext_in, int *hist) {
+    __shared__ int h[kSortBins];
     const int E = *totE;
+    if ((int)blockIdx.x * 256 >= E) return;  // workgroup-uniform
+    for (int k = threadIdx.x; k < kSortBins; k += 256) h[k] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < E; i += gridDim.x * 256)
+        atomicAdd(&h[sort_bin(sub[ext_in[i]])], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kSortBins; k += 256)
+        if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+__global__ __launch_bounds__(1024) void k_sort_scan(int *hist) {
+    __shared__ int wsum[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < kSortBins; k += 1024) hist[k] = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < E; i += 1024) atomicAdd(&hist[sort_bin(sub[ext_in[i]])], 1);
-    __syncthreads();
-    // exclusive scan: thread t owns bins 2t and 2t + 1
+    // exclusive scan in place: thread t owns bins 2t and 2t + 1
     const int h0 = hist[2 * threadIdx.x], h1 = hist[2 * threadIdx.x + 1];
     int x = h0 + h1;  // inclusive wave scan of the pair sums
     for (int off = 1; off < 64; off <<= 1) {
@@ -760,13 +773,34 @@ __global__ __launch_bounds__(1024) void k_sort_extreme(const int *sub, const int
     int before = 0;
     for (int q = 0; q < w; q++) before += wsum[q];
     const int excl = before + x - (h0 + h1);
-    __syncthreads();
     hist[2 * threadIdx.x] = excl;
     hist[2 * threadIdx.x + 1] = excl + h0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < E; i += 1024) {
-        const int idx = ext_in[i];
-        order[atomicAdd(&hist[sort_bin(sub[idx])], 1)] = idx;
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(const int *sub, const int *totE,
+                                                      const int *ext_in, int *off, int *order) {
+    __shared__ int cnt[kSortBins];  // chunk counts, then reserved bases
+    __shared__ int rank[kSortBins];
+    const int E = *totE;
+    for (int c0 = blockIdx.x * 256; c0 < E; c0 += gridDim.x * 256) {  // workgroup-uniform
+        for (int k = threadIdx.x; k < kSortBins; k += 256) {
+            cnt[k] = 0;
+            rank[k] = 0;
+        }
+        __syncthreads();
+        const int i = c0 + threadIdx.x;
+        int idx = -1, bin = 0;
+        if (i < E) {
+            idx = ext_in[i];
+            bin = sort_bin(sub[idx]);
+            atomicAdd(&cnt[bin], 1);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kSortBins; k += 256)
+            if (cnt[k]) cnt[k] = atomicAdd(&off[k], cnt[k]);
+        __syncthreads();
+        if (idx >= 0) order[cnt[bin] + atomicAdd(&rank[bin], 1)] = idx;
+        __syncthreads();
     }
 }
 
@@ -775,13 +809,17 @@ void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold,
     const int nblk = (b.n + 255) / 256;
     int *cntE = b.part, *cntH = b.part + (nblk + 1);
     hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
-                       cntE, cntH);
+                       cntE, cntH, b.sort_hist);
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntE, nblk);
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntH, nblk);
     hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
                        cntE, cntH, nblk, b.order, b.order_tmp);
-    hipLaunchKernelGGL(k_sort_extreme, dim3(1), dim3(1024), 0, s, b.sub, cntE + nblk, b.order_tmp,
-                       b.order);
+    const int sblk = std::min(1024, nblk);
+    hipLaunchKernelGGL(k_sort_hist, dim3(sblk), dim3(256), 0, s, b.sub, cntE + nblk, b.order_tmp,
+                       b.sort_hist);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, b.sort_hist);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(sblk), dim3(256), 0, s, b.sub, cntE + nblk,
+                       b.order_tmp, b.sort_hist, b.order);
 }
 
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
