@@ -65,20 +65,26 @@ __global__ __launch_bounds__(256) void k_velocity(GridParams g, F *__restrict__ 
     const F dt = (F)g.dt;
     const F rdx = (F)g.rdx;
     const F half = (F)0.5;
+    // dt / s1 where both densities are rho_inf (outside the void-fraction
+    // window, or on the first step): s1 = (rho_inf + rho_inf) / 2 = rho_inf
+    // exactly, so the quotient is the same number as in the window's path
+    const F dt_rho = dt / (((F)g.rho_inf + (F)g.rho_inf) * half);
+    const bool col_w = !first && i + 1 >= w.i0 && i <= w.i1;  // i or i + 1 in the window
     const int jend = min((int)(blockIdx.y + 1) * kRB - 1, g.M);
     for (int j = blockIdx.y * kRB + threadIdx.y; j <= jend; j += kBY) {
         const int64_t o = frame_off(i, j, g.pitch);
         const F pc = p0[o];
-        const F rc = rho_m_at<F>(g, fgn, w, first, i, j);
+        const bool in_w = col_w && j + 1 >= w.j0 && j <= w.j1;  // a density not rho_inf
+        const F rc = in_w ? rho_m_at<F>(g, fgn, w, first, i, j) : (F)g.rho_inf;
         if (j >= 1) {
-            const F s1 = (rc + rho_m_at<F>(g, fgn, w, first, i + 1, j)) * half;
             const F s2 = (-pc + p0[o + 1]) * rdx;
-            vx[o] = (vx[o] * cxa[i + 1] - dt / s1 * s2) / cxb[i + 1];
+            const F q = in_w ? dt / ((rc + rho_m_at<F>(g, fgn, w, first, i + 1, j)) * half) : dt_rho;
+            vx[o] = (vx[o] * cxa[i + 1] - q * s2) / cxb[i + 1];
         }
         if (i >= 1) {
-            const F s1 = (rc + rho_m_at<F>(g, fgn, w, first, i, j + 1)) * half;
             const F s2 = (-pc + p0[o + g.pitch]) * rdx;  // rdx, as the reference (kernel.cuh:671)
-            vy[o] = (vy[o] * cya[j + 1] - dt / s1 * s2) / cyb[j + 1];
+            const F q = in_w ? dt / ((rc + rho_m_at<F>(g, fgn, w, first, i, j + 1)) * half) : dt_rho;
+            vy[o] = (vy[o] * cya[j + 1] - q * s2) / cyb[j + 1];
         }
     }
 }
@@ -203,8 +209,13 @@ __global__ __launch_bounds__(256) void k_pressure_first(
                 fg_mp[o] = fpred;
             }
             if (i < 1 || i > g.N || j < 1 || j > g.M) continue;
-            double s5d = -fg_to_d(fpred - fold) / g.dt / (1.0 - 0.5 * fg_to_d(fpred + fold));
-            const F s5h = (F)(0.5 * s5d);
+            // outside the window fpred = fold = 0 and the source term is -0
+            F s5h = (F)(-0.0);
+            if (fpred != 0 || fold != 0) {
+                const double s5d =
+                    -fg_to_d(fpred - fold) / g.dt / (1.0 - 0.5 * fg_to_d(fpred + fold));
+                s5h = (F)(0.5 * s5d);
+            }
             F s1x;
             if (g.cylindrical)
                 s1x = (F)((-xu[i] * (double)vx[o - 1] + xu[i + 1] * (double)vx[o]) * g.rdx *

@@ -176,6 +176,35 @@ def test_deterministic_and_restore_bit_exact(brp):
     np.testing.assert_array_equal(first[0], runs[0][0])
 
 
+def test_clustered_and_lattice_scatter(brp):
+    """The RP kernel's per-wavefront LDS aggregation of the void-fraction
+    scatter: 6000 bubbles in a 2 x 2-cell cluster (thousands of contributions
+    per cell through one table slot) and a 40 x 40 lattice two cells apart
+    (every 64-bubble batch touches 256 distinct cells, so tables fill past
+    their flush point and contributions fall back to direct atomics).  f_g
+    against the oracle and bit-exact between two runs."""
+    cfg, _ = cases.bubbly_case(128, nb=10, amp_pa=0.3e6)
+    fp_x = cfg.ext_focus_x
+    fp_y = cfg.tr_focal_dist * 0.5 * np.sqrt(3.0)
+    r = np.random.default_rng(11)
+    xc = fp_x + (r.random(6000) - 0.5) * 2e-4
+    yc = fp_y + (r.random(6000) - 0.5) * 2e-4
+    gx, gy = np.meshgrid(np.arange(40) * 2e-4, np.arange(40) * 2e-4)
+    xl = fp_x - 4e-3 + gx.ravel() + 0.3e-4
+    yl = fp_y - 4e-3 + gy.ravel() + 0.3e-4
+    x, y = np.concatenate([xc, xl]), np.concatenate([yc, yl])
+    R0 = 1e-6 + 4e-6 * r.random(len(x))
+    bub = (x, y, R0, None, 1e-9)
+    g, o, gs, os_ = run_pair(brp, cfg, bub, steps=40)
+    err = report("clustered+lattice", g, o, ("p0", "fg"))
+    assert [s["n_pic"] for s in gs] == [s["n_pic"] for s in os_]
+    assert err["fg"] < 1e-10 and err["p0"] < 1e-10
+    g2 = cases.make(brp.Simulation, cfg, bub)
+    g2.step(40)
+    np.testing.assert_array_equal(g.field("fg"), g2.field("fg"))
+    np.testing.assert_array_equal(g.field("p0"), g2.field("p0"))
+
+
 def test_wide_band_and_edge_bubbles(brp):
     """smooth_delta_band = 4, bubbles straddling the domain edges and outside it."""
     cfg, _ = cases.bubbly_case(48, nb=10)
