@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "../../include/bubbles_rp.h"
+#include "rp_vtable.h"
 
 namespace brp {
 
@@ -109,27 +110,19 @@ struct RPLaunch {
     BubbleArrays b;
 };
 
-// Series coefficients of the shortened-chain sub-step's own exp and sin/cos:
-// [0, 12) 1/14!, 1/13!, ..., 1/3! (e^r - 1); [12, 20) the sine series
-// -1/17!, 1/15!, ..., 1/3!; [20, 28) the cosine series 1/18!, -1/16!, ...,
-// -1/4!.
-constexpr double kRpPoly[28] = {
-    1.0 / 87178291200.0,      1.0 / 6227020800.0,     1.0 / 479001600.0,
-    1.0 / 39916800.0,         1.0 / 3628800.0,        1.0 / 362880.0,
-    1.0 / 40320.0,            1.0 / 5040.0,           1.0 / 720.0,
-    1.0 / 120.0,              1.0 / 24.0,             1.0 / 6.0,
-    -1.0 / 355687428096000.0, 1.0 / 1307674368000.0,  -1.0 / 6227020800.0,
-    1.0 / 39916800.0,         -1.0 / 362880.0,        1.0 / 5040.0,
-    -1.0 / 120.0,             1.0 / 6.0,
-    1.0 / 6402373705728000.0, -1.0 / 20922789888000.0, 1.0 / 87178291200.0,
-    -1.0 / 479001600.0,       1.0 / 3628800.0,        -1.0 / 40320.0,
-    1.0 / 720.0,              -1.0 / 24.0};
-constexpr int kRpTable = 16 + 28;  // shared constants, one pad, kRpPoly
+// The shortened-chain sub-step's constant table: the shared constants
+// (rp_table_values, 15 doubles, padded to 64), then the piecewise polynomials
+// of the thermal-impedance functions U and W of alpha = s (1 + i)
+// (rp_vtable.h), which the kernels stage in LDS.
+constexpr int kRpVOffset = 64;
+constexpr int kRpVFunc = kVIntervals * (kVDegree + 1) * 2;  // doubles per function
+constexpr int kRpVSize = 2 * kRpVFunc;
+constexpr int kRpTable = kRpVOffset + kRpVSize;
 
 // The constants shared by every bubble in the shortened-chain RP sub-step, in
 // the field order of RPFastShared (rp_device.h): PL0, gam, mu, rho, 1/rho,
 // dt0, 3 gam, 3 (gam - 1), 3 (gam - 1) K0 T0, 2 sig, 4 mu, mu^2,
-// -2 sig rho, 4 pi K0, 16 pi mu; then (from index 16) kRpPoly.
+// -2 sig rho, 4 pi K0, 16 pi mu; then (from kRpVOffset) kVTable.
 __host__ __device__ inline void rp_table_values(const BubbleParams &bp, double v[kRpTable]) {
     v[0] = bp.PL0;
     v[1] = bp.gam;
@@ -146,8 +139,9 @@ __host__ __device__ inline void rp_table_values(const BubbleParams &bp, double v
     v[12] = -2.0 * bp.sig * bp.rho;
     v[13] = 4.0 * bp.pi * bp.K0;
     v[14] = 16.0 * bp.pi * bp.mu;
-    v[15] = 0.0;
-    for (int k = 0; k < 28; k++) v[16 + k] = kRpPoly[k];
+    for (int k = 15; k < kRpVOffset; k++) v[k] = 0.0;
+    const double *vt = &kVTable[0][0][0][0];
+    for (int k = 0; k < kRpVSize; k++) v[kRpVOffset + k] = vt[k];
 }
 
 }  // namespace brp

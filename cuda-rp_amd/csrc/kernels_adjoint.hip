@@ -490,10 +490,13 @@ __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, B
                                                  double *LRs2, double *LRs3, double *lPLp,
                                                  double *lPLn, double *lPLm, const int *sub,
                                                  unsigned int *mismatch, const int *order) {
+    __shared__ double vtab[kRpVSize];  // the U and W tables
+    rp_vtab_stage(bp.rp_shared, vtab);
+    __syncthreads();
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if ((gid >> 3) >= b.n) return;
     const int idx = order ? order[gid >> 3] : gid >> 3, d = gid & 7;
-    const RPFastShared fs = rp_shared_load(bp.rp_shared);
+    const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
     const double R0 = b.R0[idx];
     const double PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
     const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,

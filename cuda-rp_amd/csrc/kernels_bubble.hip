@@ -338,8 +338,13 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
     __shared__ WaveAgg agg_all[4];  // 25 KB: one table per wavefront
+    __shared__ double vtab[FAST ? kRpVSize : 1];  // 30 KB: the U and W tables
     WaveAgg &agg = agg_all[threadIdx.x >> 6];
     agg_clear(agg, lane);
+    if (FAST) {
+        rp_vtab_stage(rp_shared, vtab);
+        __syncthreads();
+    }
     RPShared sh{};
     if (!FAST) {
         const BubbleParams &bp = rpl(kp)->bp;
@@ -454,7 +459,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                 remain = FAST ? LF.remain : L.remain;
                 if (remain > 0.0 && !capped) {
                     if (FAST)
-                        rp_substep_g(LF, fk, rp_shared_load(rp_shared));
+                        rp_substep_g(LF, fk, rp_shared_load(rp_shared, vtab));
                     else
                         rp_substep(L, k, sh);
                     remain = FAST ? LF.remain : L.remain;

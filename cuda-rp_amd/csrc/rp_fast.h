@@ -4,14 +4,15 @@
 // Same model as rp_device.h (bubble_CUDA_kernel.cuh:227-340), rearranged for
 // latency, since one collapsing bubble's thousands of sequential sub-steps
 // bound a whole launch:
-//   * alpha = s (1 + i) always (rp_alpha), so
-//       coth(alpha) = (1 - |E|^2 + 2 i Im E) / |1 - E|^2,  E = exp(-2 alpha),
-//     from expm1(-2s) and sincos(s); for s > 20, |E| < 5e-18 and
-//     alpha coth(alpha) - 1 = (s - 1, s) to within a relative 1e-17, so the
-//     transcendentals are skipped (this is where collapsing bubbles spend
-//     about half their sub-steps);
-//   * 1 / alpha^2 = -i / (2 s^2); Lp = R (1/u + 1.5 i / s^2),
-//     u = alpha coth(alpha) - 1;
+//   * alpha = s (1 + i) always (rp_alpha), so the thermal-impedance terms
+//     are functions of s alone: Upsilon's U = (alpha coth alpha - 1)/alpha^2
+//     and Lp's W = 1/(alpha coth alpha - 1) - 3/alpha^2 (both finite at
+//     s = 0, where the two terms of W cancel).  For |alpha| <= 0.1 the
+//     reference's own Taylor series; up to s = 20 one degree-11 polynomial
+//     per interval of width 1/4 (rp_vtable.h, staged in LDS; error below
+//     1.8e-16 of |U| and |W|); for s > 20 coth alpha = 1 to below an ulp, so
+//     both follow from alpha - 1 in closed form (collapsing bubbles spend
+//     about half their sub-steps there);
 //   * omega_N and 1/omega_N from one rsqrt of eta; sqrt(c_alpha omega / R)
 //     and its reciprocal from one rsqrt;
 //   * reciprocals by v_rcp_f64 and two Newton steps (drcp), reciprocal
@@ -20,10 +21,9 @@
 //     contraction off), so the instruction sequence is fixed by the source.
 // A wavefront issues one instruction at a time whether one lane or 64 are
 // active, so the sub-step of a lone collapsing bubble costs its instruction
-// count: the polynomial coefficients are read from a constant table into
-// scalar registers right where they are used (not kept live across the
-// sub-step loop, where they would be spilled) and each Horner step is one
-// three-operand v_fma_f64.
+// count: the polynomial replaces an exp, a sin, a cos and the complex
+// arithmetic joining them (about 110 instructions per evaluation, four per
+// sub-step) by 22 fused multiply-adds on LDS operands.
 // T = double is the forward.  T = dd (value, tangent) is the adjoint's
 // linearisation: every dd operation computes its value with the same single
 // operation the double code uses, so the value part -- and with it the
@@ -33,6 +33,7 @@
 
 #include <type_traits>
 
+#include "brp_internal.h"
 #include "rp_device.h"
 
 namespace brp {
@@ -90,112 +91,56 @@ __device__ inline dd rsqrt_(dd x) {
 // Constant (scalar-loadable) memory.
 typedef __attribute__((address_space(4))) const double cdouble;
 
-// The shared constants from their device table (rp_fast_shared_values),
-// read where the sub-step uses them rather than held in scalar registers
-// across the sub-step loop.
-__device__ inline RPFastShared rp_shared_load(const double *tab) {
+// The shared constants from their device table (rp_table_values), read
+// where the sub-step uses them rather than held in scalar registers across
+// the sub-step loop; vtab is the kernel's LDS copy of the U and W tables.
+__device__ inline RPFastShared rp_shared_load(const double *tab, const double *vtab) {
     cdouble *t = (cdouble *)tab;
     asm volatile("" : "+s"(t));
     return RPFastShared{t[0], t[1], t[2],  t[3],  t[4],  t[5],  t[6], t[7],
-                        t[8], t[9], t[10], t[11], t[12], t[13], t[14], tab + 16};
+                        t[8], t[9], t[10], t[11], t[12], t[13], t[14], vtab};
 }
 
-// The series coefficients (kRpPoly), read from the table where each use
-// needs them: the pointer is made opaque per use so that the loads are
-// neither hoisted nor shared across uses (which would keep 56 scalar
-// registers live across the sub-step).
-__device__ inline cdouble *rp_poly(const double *poly) {
-    cdouble *c = (cdouble *)poly;
-    asm volatile("" : "+s"(c));
-    return c;
-}
-
-#ifndef BRP_OWN_TRANSCENDENTALS
-#define BRP_OWN_TRANSCENDENTALS 1
-#endif
-
-#if BRP_OWN_TRANSCENDENTALS
-// a b + c with c in scalar registers, as one v_fma_f64 (the compiler would
-// otherwise copy c into the destination for a two-operand v_fmac_f64).
-__device__ inline double fma_sc(double a, double b, double c) {
-    double d;
-    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
-    return d;
-}
-
-// exp(x) - 1 for -40 <= x <= -0.14 (the range alpha_coth_m1_g uses):
-// x = n ln2 + r, |r| <= ln2 / 2 (Cody-Waite reduction, ln2 in two parts),
-// e^r - 1 by its Taylor series to r^14 (truncation < 2e-19), then
-// 2^n (1 + p) - 1, which for these x is in [-1, -0.13] (no cancellation
-// beyond one ulp of 1).
-__device__ inline double expm1_neg(double x, const double *poly) {
-    cdouble *c = rp_poly(poly);
-    const double n = rint(x * 1.4426950408889634);
-    double r = fma(-n, 6.93147180369123816490e-01, x);
-    r = fma(-n, 1.90821492927058770002e-10, r);
-    double p = c[0];
-#pragma unroll
-    for (int k = 1; k < 12; k++) p = fma_sc(p, r, c[k]);
-    p = fma(p, r, 0.5);
-    p = fma(p * r, r, r);  // e^r - 1
-    return n == 0.0 ? p : fma(ldexp(1.0, (int)n), p, ldexp(1.0, (int)n) - 1.0);
-}
-
-// For 0 <= x <= 64: sa = +-sin x and sc = sin x cos x, the two quantities
-// sin 2x = 2 sc and cos 2x = 1 - 2 sa^2 are made of.  x = k pi/2 + r,
-// |r| <= pi/4 (Cody-Waite reduction, pi/2 in two parts, each step one
-// rounding through fma), Taylor series of sin r to r^17 and cos r to r^18
-// (truncation < 1e-19); sin x = +-(sin r or cos r) by the quadrant k, and
-// sin x cos x = (-1)^k sin r cos r.
-__device__ inline void trig_parts(double x, const double *poly, double *sa, double *sc) {
-    cdouble *t = rp_poly(poly);
-    const double k = rint(x * 0.63661977236758134308);
-    double r = fma(-k, 1.57079632679489655800e+00, x);
-    r = fma(-k, 6.12323399573676603587e-17, r);
-    const double r2 = r * r;
-    double ps = t[12];
-#pragma unroll
-    for (int q = 13; q < 20; q++) ps = fma_sc(ps, r2, t[q]);
-    ps = fma(-(ps * r2), r, r);  // r - r^3/6 + ...
-    double pc = t[20];
-#pragma unroll
-    for (int q = 21; q < 28; q++) pc = fma_sc(pc, r2, t[q]);
-    pc = fma(pc, r2, 0.5);
-    pc = fma(-pc, r2, 1.0);  // 1 - r^2/2 + ...
-    const bool odd = (((int)k) & 1) != 0;
-    const double p = ps * pc;
-    *sa = odd ? pc : ps;
-    *sc = odd ? -p : p;
-}
-__device__ inline double expm1_(double x, const double *poly) { return expm1_neg(x, poly); }
-__device__ inline void trig_parts_(double x, const double *poly, double *sa, double *sc) {
-    trig_parts(x, poly, sa, sc);
-}
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) const double ldouble;  // LDS
 #else
-__device__ inline double expm1_(double x, const double *) { return expm1(x); }
-__device__ inline void trig_parts_(double x, const double *, double *sa, double *sc) {
-    double sn, cn;
-    sincos(x, &sn, &cn);
-    *sa = sn;
-    *sc = sn * cn;
-}
+typedef const double ldouble;  // the host pass only parses device code
 #endif
-__device__ inline dd expm1_(dd x, const double *poly) {
-    const double m = expm1_(x.v, poly);
-    return dd(m, (m + 1.0) * x.d);
+
+// Copies the U and W tables (kRpVSize doubles from the device table) into the
+// workgroup's LDS array; the caller synchronises the workgroup afterwards.
+__device__ inline void rp_vtab_stage(const double *tab, double *lds) {
+    for (int k = threadIdx.x; k < kRpVSize; k += blockDim.x) lds[k] = tab[kRpVOffset + k];
 }
-// The tangent form: sa = sin x with its tangent cos x dx (the value may
-// differ from the double form's in sign only, which no caller sees: they use
-// sa^2 and sc), sc = sin x cos x with value equal to the double form's.
-__device__ inline void trig_parts_(dd x, const double *poly, dd *sa, dd *sc) {
-    double a, p;
-    trig_parts_(x.v, poly, &a, &p);
-    // sin x and cos x up to a common sign pattern: a = +-sin x, and
-    // p / a = cos x when a = sin x; recover cos x from a and p without a
-    // division by choosing the tangent of sa consistently with its sign
-    const double c = a != 0.0 ? p / a : 1.0;  // cos x * (sign of sa / sign of sin x)
-    *sa = dd(a, c * x.d);
-    *sc = dd(p, (c * c - a * a) * x.d);
+
+struct cx_d {
+    double re, im;
+};
+
+// U or W (ftab = that function's table in LDS) for 0 <= s <= 20: interval
+// k = floor(4 s), t = 8 s - (2k + 1), Horner in t for the real and imaginary
+// parts (two independent chains); with dpr/dpi also d/ds.
+__device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullptr,
+                              double *dpi = nullptr) {
+    const int k = max(0, min((int)(s * 4.0), kVIntervals - 1));
+    const double t = fma(s, 8.0, -(2.0 * k + 1.0));
+    ldouble *c = (ldouble *)ftab + k * (2 * (kVDegree + 1));
+    double pr = c[2 * kVDegree], pi = c[2 * kVDegree + 1];
+    double qr = 0.0, qi = 0.0;  // d/dt
+#pragma unroll
+    for (int j = kVDegree - 1; j >= 0; j--) {
+        if (dpr) {
+            qr = fma(qr, t, pr);
+            qi = fma(qi, t, pi);
+        }
+        pr = fma(pr, t, c[2 * j]);
+        pi = fma(pi, t, c[2 * j + 1]);
+    }
+    if (dpr) {
+        *dpr = 8.0 * qr;  // dt/ds = 8
+        *dpi = 8.0 * qi;
+    }
+    return cx_d{pr, pi};
 }
 
 template <typename T>
@@ -203,72 +148,69 @@ struct cx {
     T re, im;
 };
 
-// alpha coth(alpha) - 1 = (re + i im) / d for alpha = s (1 + i), s > 0,
-// with the division left to the caller, which folds it into its own
-// reciprocal (one reciprocal per caller instead of two).  The callers use
-// it for s > 0.1 / sqrt(2) only, so expm1_ sees -40 <= -2s < -0.14.
-template <typename T>
-struct cxd {
-    T re, im, d;
-};
-template <typename T>
-__device__ inline cxd<T> alpha_coth_m1_g(T s, const double *poly) {
-    if (val(s) > 20.0) return cxd<T>{s - T(1.0), s, T(1.0)};
-    const T m = expm1_(-2.0 * s, poly);  // exp(-2s) - 1
-    T sa, sc;                            // +-sin s, sin s cos s
-    trig_parts_(s, poly, &sa, &sc);
-    const T e2 = T(1.0) + m;
-    const T s2 = 2.0 * sc;                     // sin 2s
-    const T c2 = fma_(-2.0 * sa, sa, T(1.0));  // cos 2s
-    const T ei = -(e2 * s2);                   // Im E
-    const T one_m_er = fma_(2.0 * sa, sa, -(m * c2));  // 1 - Re E
-    const T D = fma_(one_m_er, one_m_er, ei * ei);     // |1 - E|^2
-    const T nr = -(m * (m + T(2.0)));                  // 1 - |E|^2
-    const T ni = 2.0 * ei;
-    // coth(alpha) = (nr + i ni) / D
-    return cxd<T>{fma_(s, nr - ni, -D), s * (nr + ni), D};
+__device__ inline cx<double> v_table(double s, const double *ftab) {
+    const cx_d v = v_poly(s, ftab);
+    return cx<double>{v.re, v.im};
+}
+// The tangent form: the value is the double form's, the tangent d/ds ds.
+__device__ inline cx<dd> v_table(dd s, const double *ftab) {
+    double dr, di;
+    const cx_d v = v_poly(s.v, ftab, &dr, &di);
+    return cx<dd>{dd(v.re, dr * s.d), dd(v.im, di * s.d)};
 }
 
-// Upsilon(alpha) (bubble_CUDA_kernel.cuh:296-311) from s and 1/s:
-// 3 gam / (1 + k3 (alpha coth alpha - 1) / alpha^2) with 1/alpha^2 =
-// -i / (2 s^2), written as 3 gam D (A - i B) / (A^2 + B^2).
+// |alpha| <= 0.1: the reference's Taylor branches (bubble_CUDA_kernel.cuh:
+// 302-306, 317-319).  They are kept rather than the tables because there
+// omega, and with it s^2, can be as small as the eta floor allows, and the
+// sub-step divides the imaginary parts of Upsilon and Lp (of order s^2) by
+// omega: the series give them to relative accuracy, the tables only to
+// accuracy relative to |U| and |W|.
 template <typename T>
-__device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3, const double *poly) {
-    T A, B, D;
-    if (val(s) * 1.4142135623730951 > 1.0e-1) {
-        const cxd<T> u = alpha_coth_m1_g(s, poly);
-        const T q = k3 * (0.5 * (is * is));  // k3 / (2 s^2)
-        A = fma_(q, u.im, u.d);
-        B = -(q * u.re);
-        D = u.d;
-    } else {  // 1/3 + a^2 (-1/45 + a^2 (2/945 - a^2/4725)),  a^2 = i q
+__device__ inline bool small_alpha(T s) {
+    return !(val(s) * 1.4142135623730951 > 1.0e-1);
+}
+
+// Upsilon(alpha) = 3 gam / (1 + k3 U) (bubble_CUDA_kernel.cuh:296-311) from
+// s and is = 1/s, U = (alpha coth alpha - 1) / alpha^2: the table up to
+// s = 20; beyond it coth alpha = 1 to below an ulp, so
+// U = (alpha - 1) / alpha^2 = (is / 2) (1 + i (is - 1)).
+template <typename T>
+__device__ inline cx<T> upsilon_g(T s, T is, double gam, double k3, const double *vtab) {
+    cx<T> u;
+    if (val(s) > 20.0) {
+        const T h = 0.5 * is;
+        u = cx<T>{h, h * (is - T(1.0))};
+    } else if (small_alpha(s)) {  // 1/3 + a^2 (-1/45 + a^2 (2/945 - a^2/4725)), a^2 = i q
         const T q = 2.0 * (s * s);
-        T ur = T(2.0 / 945.0), ui = -(q * (1.0 / 4725.0));
-        T nr = fma_(-q, ui, T(-1.0 / 45.0)), ni = q * ur;
-        const T tr = fma_(-q, ni, T(1.0 / 3.0));
-        const T ti = q * nr;
-        A = fma_(T(k3), tr, T(1.0));
-        B = k3 * ti;
-        D = T(1.0);
+        const T ur = T(2.0 / 945.0), ui = -(q * (1.0 / 4725.0));
+        const T nr = fma_(-q, ui, T(-1.0 / 45.0)), ni = q * ur;
+        u = cx<T>{fma_(-q, ni, T(1.0 / 3.0)), q * nr};
+    } else {
+        u = v_table(s, vtab);
     }
-    const T f = ((3.0 * gam) * D) * rcp_(fma_(A, A, B * B));
-    return cx<T>{A * f, -(B * f)};
+    const T dr = fma_(T(k3), u.re, T(1.0)), di = k3 * u.im;
+    const T f = (3.0 * gam) * rcp_(fma_(dr, dr, di * di));
+    return cx<T>{dr * f, -(di * f)};
 }
 
-// solveLp (bubble_CUDA_kernel.cuh:314-320) from s and 1/s:
-// R (1 / (alpha coth alpha - 1) - 3 / alpha^2).
+// solveLp (bubble_CUDA_kernel.cuh:314-320), Lp = R W with
+// W = 1 / (alpha coth alpha - 1) - 3 / alpha^2: the table up to s = 20;
+// beyond it W = 1 / ((s - 1) + i s) + 1.5 i is^2.
 template <typename T>
-__device__ inline cx<T> lp_g(T s, T is, T R, const double *poly) {
-    if (val(s) * 1.4142135623730951 > 1.0e-1) {
-        const cxd<T> u = alpha_coth_m1_g(s, poly);
-        const T g = u.d * rcp_(fma_(u.re, u.re, u.im * u.im));
-        return cx<T>{R * (u.re * g), R * fma_(1.5 * is, is, -(u.im * g))};
+__device__ inline cx<T> lp_g(T s, T is, T R, const double *vtab) {
+    if (val(s) > 20.0) {
+        const T sm = s - T(1.0);
+        const T id = rcp_(fma_(sm, sm, s * s));
+        return cx<T>{R * (sm * id), R * fma_(T(1.5) * is, is, -(s * id))};
     }
-    // 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875)),  a^2 = i q
-    const T q = 2.0 * (s * s);
-    T ur = T(2.0 / 7875.0), ui = -(q * (37.0 / 3031875.0));
-    T nr = fma_(-q, ui, T(-1.0 / 175.0)), ni = q * ur;
-    return cx<T>{R * fma_(-q, ni, T(1.0 / 5.0)), R * (q * nr)};
+    if (small_alpha(s)) {  // 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875))
+        const T q = 2.0 * (s * s);
+        const T ur = T(2.0 / 7875.0), ui = -(q * (37.0 / 3031875.0));
+        const T nr = fma_(-q, ui, T(-1.0 / 175.0)), ni = q * ur;
+        return cx<T>{R * fma_(-q, ni, T(1.0 / 5.0)), R * (q * nr)};
+    }
+    const cx<T> w = v_table(s, vtab + kRpVFunc);
+    return cx<T>{R * w.re, R * w.im};
 }
 
 // omega = sqrt(max(eta, floor)) / (rho R) and 1 / omega from one rsqrt.
@@ -363,7 +305,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     for (int it = 0; it < 3; it++) {
         const T x = ca * om;
         const T isa = rsqrt_(x);
-        const cx<T> ups = upsilon_g(x * isa, isa, s.gam, s.k3, s.poly);
+        const cx<T> ups = upsilon_g(x * isa, isa, s.gam, s.k3, s.vtab);
         const T ur = ups.re * coef1, ui = ups.im * coef1;
         const T mu_eff = fma_(ui * PGn, 0.25 * iom, T(s.mu));  // mu + ui PG / (4 omega)
         eta = fma_(ur, coef2, fma_(-(mu_eff * mu_eff), coef3, value1));
@@ -371,7 +313,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     }
     const T x = ca * om;
     const T isa = rsqrt_(x);
-    const cx<T> Lp = lp_g(x * isa, isa, Rn, s.poly);
+    const cx<T> Lp = lp_g(x * isa, isa, Rn, s.vtab);
     // solvePG (bubble_CUDA_kernel.cuh:322-340)
     const T Rbar = 0.5 * (Rp + Rn);
     const T iRbar = rcp_(Rbar);
