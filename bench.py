@@ -367,8 +367,8 @@ def main():
                      "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both "
                              "78.6 TFLOP/s.  A launch is bound by the longest bubbles' "
                              "sequential sub-step chains, each on a wavefront of its own "
-                             "that issues one instruction per 4 cycles whatever its active "
-                             "lanes (about 1000 instructions per sub-step), not by FP64 "
+                             "that issues at most one instruction per 4 cycles whatever its "
+                             "active lanes (about 700 instructions per sub-step), not by FP64 "
                              "throughput; traffic = HBM bytes per launch from rocprofv3 "
                              "FETCH_SIZE / WRITE_SIZE (profiles/)"},
         "grid_roofline": grid_roofline(ph_m, n, args.steps, mixed=True),

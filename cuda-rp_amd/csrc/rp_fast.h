@@ -8,9 +8,9 @@
 //     are functions of s alone: Upsilon's U = (alpha coth alpha - 1)/alpha^2
 //     and Lp's W = 1/(alpha coth alpha - 1) - 3/alpha^2 (both finite at
 //     s = 0, where the two terms of W cancel).  For |alpha| <= 0.1 the
-//     reference's own Taylor series; up to s = 20 one degree-11 polynomial
-//     per interval of width 1/4 (rp_vtable.h, staged in LDS; error below
-//     1.8e-16 of |U| and |W|); for s > 20 coth alpha = 1 to below an ulp, so
+//     reference's own Taylor series; up to s = 20 one degree-9 polynomial
+//     per interval of width 1/8 (rp_vtable.h, staged in LDS; error below
+//     2e-16 of |U| and |W|); for s > 20 coth alpha = 1 to below an ulp, so
 //     both follow from alpha - 1 in closed form (collapsing bubbles spend
 //     about half their sub-steps there);
 //   * omega_N and 1/omega_N from one rsqrt of eta; sqrt(c_alpha omega / R)
@@ -23,7 +23,7 @@
 // active, so the sub-step of a lone collapsing bubble costs its instruction
 // count: the polynomial replaces an exp, a sin, a cos and the complex
 // arithmetic joining them (about 110 instructions per evaluation, four per
-// sub-step) by 22 fused multiply-adds on LDS operands.
+// sub-step) by 18 fused multiply-adds on LDS operands.
 // T = double is the forward.  T = dd (value, tangent) is the adjoint's
 // linearisation: every dd operation computes its value with the same single
 // operation the double code uses, so the value part -- and with it the
@@ -118,12 +118,12 @@ struct cx_d {
 };
 
 // U or W (ftab = that function's table in LDS) for 0 <= s <= 20: interval
-// k = floor(4 s), t = 8 s - (2k + 1), Horner in t for the real and imaginary
-// parts (two independent chains); with dpr/dpi also d/ds.
+// k = floor(8 s), t = 16 s - (2k + 1), Horner in t for the real and
+// imaginary parts (two independent chains); with dpr/dpi also d/ds.
 __device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullptr,
                               double *dpi = nullptr) {
-    const int k = max(0, min((int)(s * 4.0), kVIntervals - 1));
-    const double t = fma(s, 8.0, -(2.0 * k + 1.0));
+    const int k = max(0, min((int)(s * double(kVPerUnit)), kVIntervals - 1));
+    const double t = fma(s, 2.0 * kVPerUnit, -(2.0 * k + 1.0));
     ldouble *c = (ldouble *)ftab + k * (2 * (kVDegree + 1));
     double pr = c[2 * kVDegree], pi = c[2 * kVDegree + 1];
     double qr = 0.0, qi = 0.0;  // d/dt
@@ -137,8 +137,8 @@ __device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullpt
         pi = fma(pi, t, c[2 * j + 1]);
     }
     if (dpr) {
-        *dpr = 8.0 * qr;  // dt/ds = 8
-        *dpi = 8.0 * qi;
+        *dpr = (2.0 * kVPerUnit) * qr;  // dt/ds
+        *dpi = (2.0 * kVPerUnit) * qi;
     }
     return cx_d{pr, pi};
 }

@@ -2,7 +2,7 @@
 (cuda-rp_amd/csrc/rp_vtable.h, read by rp_fast.h) against the oracle's
 Upsilon and solveLp (bubble_CUDA_kernel.cuh:296-323) and against closed
 forms.  CPU only: the header is parsed and evaluated in Python the way the
-kernel evaluates it (interval k = floor(4 s), Horner in t = 8 s - (2k + 1)).
+kernel evaluates it (interval k = floor(8 s), Horner in t = 16 s - (2k + 1)).
 
   U(s) = (alpha coth alpha - 1) / alpha^2,  Upsilon = 3 gam / (1 + 3 (gam - 1) U)
   W(s) = 1 / (alpha coth alpha - 1) - 3 / alpha^2,  Lp = R W,
@@ -26,6 +26,8 @@ HEADER = os.path.join(CSRC, "rp_vtable.h")
 
 def _table():
     txt = open(HEADER).read()
+    global PER_UNIT
+    PER_UNIT = int(re.search(r"kVPerUnit = (\d+);", txt).group(1))
     nint = int(re.search(r"kVIntervals = (\d+);", txt).group(1))
     deg = int(re.search(r"kVDegree = (\d+);", txt).group(1))
     body = txt[txt.index("kVTable[2]"):]
@@ -39,8 +41,8 @@ NINT, DEG = TAB.shape[1], TAB.shape[2] - 1
 
 
 def table_eval(f, s):
-    k = max(0, min(int(s * 4.0), NINT - 1))
-    t = s * 8.0 - (2.0 * k + 1.0)
+    k = max(0, min(int(s * PER_UNIT), NINT - 1))
+    t = s * (2.0 * PER_UNIT) - (2.0 * k + 1.0)
     c = TAB[f, k]
     pr, pi = c[DEG]
     for j in range(DEG - 1, -1, -1):
@@ -58,8 +60,8 @@ def w_oracle(s):
 
 
 def test_table_shape_covers_zero_to_twenty():
-    assert TAB.shape == (2, 80, 12, 2)
-    assert NINT / 4.0 == 20.0
+    assert TAB.shape == (2, 160, 10, 2)
+    assert NINT / PER_UNIT == 20.0
 
 
 def test_values_at_zero():
