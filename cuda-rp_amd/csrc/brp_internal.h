@@ -112,7 +112,7 @@ struct RPLaunch {
 
 // The shortened-chain sub-step's constant table: the shared constants
 // (rp_table_values, 15 doubles, padded to 64), then the piecewise polynomials
-// of the thermal-impedance functions U and W of alpha = s (1 + i)
+// of the thermal-impedance functions U and Y of alpha = s (1 + i)
 // (rp_vtable.h), which the kernels stage in LDS.
 constexpr int kRpVOffset = 64;
 constexpr int kRpVFunc = kVIntervals * (kVDegree + 1) * 2;  // doubles per function

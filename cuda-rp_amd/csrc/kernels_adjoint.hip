@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, B
                                                  double *LRs2, double *LRs3, double *lPLp,
                                                  double *lPLn, double *lPLm, const int *sub,
                                                  unsigned int *mismatch, const int *order) {
-    __shared__ double vtab[kRpVSize];  // the U and W tables
+    __shared__ double vtab[kRpVSize];  // the U and Y tables
     rp_vtab_stage(bp.rp_shared, vtab);
     __syncthreads();
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;

@@ -338,7 +338,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
     __shared__ WaveAgg agg_all[4];  // 25 KB: one table per wavefront
-    __shared__ double vtab[FAST ? kRpVSize : 1];  // 50 KB: the U and W tables
+    __shared__ double vtab[FAST ? kRpVSize : 1];  // 50 KB: the U and Y tables
     WaveAgg &agg = agg_all[threadIdx.x >> 6];
     agg_clear(agg, lane);
     if (FAST) {

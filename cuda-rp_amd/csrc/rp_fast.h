@@ -6,11 +6,12 @@
 // bound a whole launch:
 //   * alpha = s (1 + i) always (rp_alpha), so the thermal-impedance terms
 //     are functions of s alone: Upsilon's U = (alpha coth alpha - 1)/alpha^2
-//     and Lp's W = 1/(alpha coth alpha - 1) - 3/alpha^2 (both finite at
+//     and Lp's W = 1/(alpha coth alpha - 1) - 3/alpha^2, of which solvePG
+//     needs only Lp/|Lp|^2 = Y/R, Y = 1/conj(W) (U and Y are finite at
 //     s = 0, where the two terms of W cancel).  For |alpha| <= 0.1 the
 //     reference's own Taylor series; up to s = 20 one degree-9 polynomial
 //     per interval of width 1/8 (rp_vtable.h, staged in LDS; error below
-//     2e-16 of |U| and |W|); for s > 20 coth alpha = 1 to below an ulp, so
+//     2e-16 of |U| and |Y|); for s > 20 coth alpha = 1 to below an ulp, so
 //     both follow from alpha - 1 in closed form (collapsing bubbles spend
 //     about half their sub-steps there);
 //   * omega_N and 1/omega_N from one rsqrt of eta; sqrt(c_alpha omega / R)
@@ -93,7 +94,7 @@ typedef __attribute__((address_space(4))) const double cdouble;
 
 // The shared constants from their device table (rp_table_values), read
 // where the sub-step uses them rather than held in scalar registers across
-// the sub-step loop; vtab is the kernel's LDS copy of the U and W tables.
+// the sub-step loop; vtab is the kernel's LDS copy of the U and Y tables.
 __device__ inline RPFastShared rp_shared_load(const double *tab, const double *vtab) {
     cdouble *t = (cdouble *)tab;
     asm volatile("" : "+s"(t));
@@ -107,7 +108,7 @@ typedef __attribute__((address_space(3))) const double ldouble;  // LDS
 typedef const double ldouble;  // the host pass only parses device code
 #endif
 
-// Copies the U and W tables (kRpVSize doubles from the device table) into the
+// Copies the U and Y tables (kRpVSize doubles from the device table) into the
 // workgroup's LDS array; the caller synchronises the workgroup afterwards.
 __device__ inline void rp_vtab_stage(const double *tab, double *lds) {
     for (int k = threadIdx.x; k < kRpVSize; k += blockDim.x) lds[k] = tab[kRpVOffset + k];
@@ -117,7 +118,7 @@ struct cx_d {
     double re, im;
 };
 
-// U or W (ftab = that function's table in LDS) for 0 <= s <= 20: interval
+// U or Y (ftab = that function's table in LDS) for 0 <= s <= 20: interval
 // k = floor(8 s), t = 16 s - (2k + 1), Horner in t for the real and
 // imaginary parts (two independent chains); with dpr/dpi also d/ds.
 __device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullptr,
@@ -163,9 +164,9 @@ __device__ inline cx<dd> v_table(dd s, const double *ftab) {
 // |alpha| <= 0.1: the reference's Taylor branches (bubble_CUDA_kernel.cuh:
 // 302-306, 317-319).  They are kept rather than the tables because there
 // omega, and with it s^2, can be as small as the eta floor allows, and the
-// sub-step divides the imaginary parts of Upsilon and Lp (of order s^2) by
-// omega: the series give them to relative accuracy, the tables only to
-// accuracy relative to |U| and |W|.
+// sub-step divides the imaginary parts of Upsilon and Lp / |Lp|^2 (of order
+// s^2) by omega: the series give them to relative accuracy, the tables only
+// to accuracy relative to |U| and |Y|.
 template <typename T>
 __device__ inline bool small_alpha(T s) {
     return !(val(s) * 1.4142135623730951 > 1.0e-1);
@@ -194,24 +195,32 @@ __device__ inline cx<T> upsilon_g(T s, T is, double g3, double k3, const double 
     return cx<T>{dr * f, -(di * f)};
 }
 
-// solveLp (bubble_CUDA_kernel.cuh:314-320), Lp = R W with
-// W = 1 / (alpha coth alpha - 1) - 3 / alpha^2: the table up to s = 20;
-// beyond it W = 1 / ((s - 1) + i s) + 1.5 i is^2.
+// solveLp (bubble_CUDA_kernel.cuh:314-320) as solvePG uses it
+// (bubble_CUDA_kernel.cuh:328-329): Lp / |Lp|^2 = Y / R with Lp = R W,
+// W = 1 / (alpha coth alpha - 1) - 3 / alpha^2 and Y = 1 / conj(W); iR = 1/R.
+// The table up to s = 20; beyond it W = 1/(alpha - 1) - 3/alpha^2, so
+// Y = conj(N) / conj(D) with N = alpha^2 (alpha - 1), D = alpha^2 - 3 alpha + 3,
+// both divided by s^2: conj(N)/s^2 = -2s - 2i(s - 1),
+// conj(D)/s^2 = 3 is (is - 1) - i (2 - 3 is).
 template <typename T>
-__device__ inline cx<T> lp_g(T s, T is, T R, const double *vtab) {
+__device__ inline cx<T> lp_inv_g(T s, T is, T iR, const double *vtab) {
+    cx<T> y;
     if (val(s) > 20.0) {
-        const T sm = s - T(1.0);
-        const T id = rcp_(fma_(sm, sm, s * s));
-        return cx<T>{R * (sm * id), R * fma_(T(1.5) * is, is, -(s * id))};
-    }
-    if (small_alpha(s)) {  // 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875))
+        const T a = -2.0 * s, b = -2.0 * (s - T(1.0));
+        const T c = (3.0 * is) * (is - T(1.0)), d = fma_(T(3.0), is, T(-2.0));
+        const T id = rcp_(fma_(c, c, d * d));
+        y = cx<T>{fma_(a, c, b * d) * id, fma_(b, c, -(a * d)) * id};
+    } else if (small_alpha(s)) {  // W = 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875))
         const T q = 2.0 * (s * s);
         const T ur = T(2.0 / 7875.0), ui = -(q * (37.0 / 3031875.0));
         const T nr = fma_(-q, ui, T(-1.0 / 175.0)), ni = q * ur;
-        return cx<T>{R * fma_(-q, ni, T(1.0 / 5.0)), R * (q * nr)};
+        const T wr = fma_(-q, ni, T(1.0 / 5.0)), wi = q * nr;
+        const T iw = rcp_(fma_(wr, wr, wi * wi));
+        y = cx<T>{wr * iw, wi * iw};
+    } else {
+        y = v_table(s, vtab + kRpVFunc);
     }
-    const cx<T> w = v_table(s, vtab + kRpVFunc);
-    return cx<T>{R * w.re, R * w.im};
+    return cx<T>{y.re * iR, y.im * iR};
 }
 
 // omega = sqrt(max(eta, floor)) / (rho R) and 1 / omega from one rsqrt.
@@ -314,13 +323,12 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     }
     const T x = ca * om;
     const T isa = rsqrt_(x);
-    const cx<T> Lp = lp_g(x * isa, isa, Rn, s.vtab);
+    const cx<T> Ln = lp_inv_g(x * isa, isa, iRn, s.vtab);  // Lp / |Lp|^2
     // solvePG (bubble_CUDA_kernel.cuh:322-340)
     const T Rbar = 0.5 * (Rp + Rn);
     const T iRbar = rcp_(Rbar);
     const T Rx_ = Rbar * k.iR0, Rpx = Rp * k.iR0, Rnx = Rn * k.iR0;
-    const T iL = rcp_(fma_(Lp.re, Lp.re, Lp.im * Lp.im));
-    const T Lnr = Lp.re * iL, Lni = Lp.im * iL;
+    const T Lnr = Ln.re, Lni = Ln.im;
     const T cdt = (s.C0 * iRbar) * k.iPG0;  // c0 / dt
     const T c0 = dtL * cdt;
     const T c3 = fma_(c0 * Lnr, 0.5 * (Rx_ * Rx_ * Rx_), ((1.5 * s.gam) * (Rp - Rn)) * iRbar);
@@ -332,10 +340,10 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     if constexpr (std::is_same<T, double>::value) {
         // heat and viscous dissipation (Q_B): dTdr dt with the 1/dt of its
         // second term cancelled against the dt factor
-        const double s1 = iL * k.T0_PG0R03;
+        // Lp.re / |Lp|^2 = Lnr, Lp.im / |Lp|^2 = Lni
         const double dTdr_dt =
-            fma(Lp.re * s1 * dtL, fma(-0.5 * (PGp + PGn), Rbar * Rbar * Rbar, k.PG0R03),
-                (Lp.im * s1 * iom) * fma(PGp, Rp * Rp * Rp, -(PGn * (Rn * Rn * Rn))));
+            fma(Lnr * k.T0_PG0R03 * dtL, fma(-0.5 * (PGp + PGn), Rbar * Rbar * Rbar, k.PG0R03),
+                (Lni * k.T0_PG0R03 * iom) * fma(PGp, Rp * Rp * Rp, -(PGn * (Rn * Rn * Rn))));
         L.SumHeat = fma(-s.four_pi_K0 * (Rp * Rp), dTdr_dt, L.SumHeat);
         L.SumVis = fma(s.sixteen_pi_mu * Rp * dtL, d1R * d1R, L.SumVis);
     }
