@@ -127,7 +127,20 @@ __device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullpt
     const int k = max(0, min((int)ldexp(s, 3), kVIntervals - 1));
     const double t = ldexp(s, 4) - (2.0 * k + 1.0);  // exact product, one rounding
     ldouble *c = (ldouble *)ftab + k * (2 * (kVDegree + 1));
-    double pr = c[2 * kVDegree], pi = c[2 * kVDegree + 1];
+    double cr[kVDegree + 1], ci[kVDegree + 1];
+#pragma unroll
+    for (int j = kVDegree; j >= 0; j--) {
+        cr[j] = c[2 * j];
+        ci[j] = c[2 * j + 1];
+    }
+    if (!dpr) {
+        // all ten LDS reads first, then the two Horner chains: the chains
+        // wait only for the first read's latency (the scheduler otherwise
+        // interleaves reads and waits along the chain)
+        __builtin_amdgcn_sched_group_barrier(0x0100, kVDegree + 1, 0);  // DS reads
+        __builtin_amdgcn_sched_group_barrier(0x0002, 2 * kVDegree, 0);  // VALU
+    }
+    double pr = cr[kVDegree], pi = ci[kVDegree];
     double qr = 0.0, qi = 0.0;  // d/dt
 #pragma unroll
     for (int j = kVDegree - 1; j >= 0; j--) {
@@ -135,8 +148,8 @@ __device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullpt
             qr = fma(qr, t, pr);
             qi = fma(qi, t, pi);
         }
-        pr = fma(pr, t, c[2 * j]);
-        pi = fma(pi, t, c[2 * j + 1]);
+        pr = fma(pr, t, cr[j]);
+        pi = fma(pi, t, ci[j]);
     }
     if (dpr) {
         *dpr = (2.0 * kVPerUnit) * qr;  // dt/ds
@@ -167,9 +180,12 @@ __device__ inline cx<dd> v_table(dd s, const double *ftab) {
 // sub-step divides the imaginary parts of Upsilon and Lp / |Lp|^2 (of order
 // s^2) by omega: the series give them to relative accuracy, the tables only
 // to accuracy relative to |U| and |Y|.
+// |alpha| = sqrt(2) s, so |alpha| > 0.1 is s > 0.1 / sqrt(2) (one literal;
+// the two forms can differ only within an ulp of the threshold, where the
+// series and the table agree to the series' truncation error).
 template <typename T>
 __device__ inline bool small_alpha(T s) {
-    return !(val(s) * 1.4142135623730951 > 1.0e-1);
+    return !(val(s) > 0.070710678118654752);
 }
 
 // Upsilon(alpha) = 3 gam / (1 + k3 U) (bubble_CUDA_kernel.cuh:296-311) from
