@@ -341,9 +341,21 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     __shared__ double vtab[FAST ? kRpVSize : 1];  // 50 KB: the U and Y tables
     WaveAgg &agg = agg_all[threadIdx.x >> 6];
     agg_clear(agg, lane);
+    // The shared RP constants go through LDS into vector registers, held for
+    // the whole kernel (245 VGPRs, two wavefronts per SIMD still fit): read
+    // from the constant table instead, each sub-step waited on its scalar
+    // loads (no scalar registers are free to hold them).
+    __shared__ double shc[16];
     if (FAST) {
         rp_vtab_stage(rp_shared, vtab);
+        if (threadIdx.x < 15) shc[threadIdx.x] = rp_shared[threadIdx.x];
         __syncthreads();
+    }
+    RPFastShared fsh{};
+    if (FAST) {
+        const ldouble *t = (const ldouble *)shc;
+        fsh = RPFastShared{t[0], t[1], t[2],  t[3],  t[4],  t[5],  t[6], t[7],
+                           t[8], t[9], t[10], t[11], t[12], t[13], t[14], vtab};
     }
     RPShared sh{};
     if (!FAST) {
@@ -459,7 +471,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                 remain = FAST ? LF.remain : L.remain;
                 if (remain > 0.0 && !capped) {
                     if (FAST)
-                        rp_substep_g(LF, fk, rp_shared_load(rp_shared, vtab));
+                        rp_substep_g(LF, fk, fsh);
                     else
                         rp_substep(L, k, sh);
                     remain = FAST ? LF.remain : L.remain;
