@@ -334,7 +334,8 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                                                             int plm, int pln, int plp,
                                                             const F *__restrict__ p0,
                                                             int64_t *__restrict__ fg_target,
-                                                            StepScalars *sc, int slot) {
+                                                            StepScalars *sc, int slot,
+                                                            int solo_cap) {
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
     __shared__ WaveAgg agg_all[4];  // 25 KB: one table per wavefront
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     {
         const BubbleArrays &b = rpl(kp)->b;
         nb = b.n;
-        E = min(b.part[(b.n + 255) / 256], (int)gridDim.x);
+        E = min(min(b.part[(b.n + 255) / 256], (int)gridDim.x), solo_cap);
     }
     int idx = -1;                  // bubble of this lane, -1 when idle
     bool pre = false;
@@ -646,12 +647,22 @@ void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev
     if (host.b.n <= 0) return;
     const dim3 grid(bubble_step_blocks(host.b.n, lean));
     const int cap = host.bp.max_substeps;
+    // Dedicated wavefronts: up to one per workgroup on the lean grid, where
+    // the longest chains bound the launch; g_cus / 8 on the full grid, which
+    // is chosen when the total work bounds it: a wavefront with one active
+    // lane still takes its SIMD's issue slots, so every dedicated wavefront
+    // costs a SIMD of throughput there.  4096^2, 2^22 clustered bubbles
+    // (full grid): 512 dedicated 49.8, 128 46.8, 32 46.4, 0 46.2 ms/step;
+    // 2048^2 (lean grid): 256 17.2, 32 17.4, 0 19.1 ms/step.
+    const int solo_cap = lean ? (int)grid.x : std::max(1, g_cus / 8);
     if (host.bp.rp_exact)
         hipLaunchKernelGGL((k_bubble_persistent<F, false>), grid, dim3(256), 0, s, dev, cap,
-                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot);
+                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot,
+                           solo_cap);
     else
         hipLaunchKernelGGL((k_bubble_persistent<F, true>), grid, dim3(256), 0, s, dev, cap,
-                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot);
+                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot,
+                           solo_cap);
 }
 
 // Work-queue order (longest-first scheduling: long sequential sub-step
