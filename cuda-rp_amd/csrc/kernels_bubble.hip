@@ -328,6 +328,10 @@ __device__ inline crpl *rpl(const RPLaunch *p) {
     return q;
 }
 
+// Finished lanes a wavefront collects before it leaves the sub-step loop to
+// write them back and refill them.
+constexpr int kRefillBatch = 24;
+
 template <typename F, bool FAST>
 __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, int max_substeps,
                                                             const double *rp_shared, int cur,
@@ -459,9 +463,15 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         }
         const bool active = idx >= 0;
         if (__ballot(active) == 0ull) break;
-        // ---- sub-steps until some lane's bubble reaches the sync time (or
-        // the sub-step cap); the refill and write-back code stays outside
-        // this loop, and with it the values only that code needs
+        // ---- sub-steps until kRefillBatch lanes' bubbles have reached the
+        // sync time (or the sub-step cap), or no active lane is left
+        // running; finished lanes idle meanwhile.  The refill and write-back
+        // code stays outside this loop, and with it the values only that
+        // code needs.  Refilling as soon as one lane finished made every
+        // wave with a long chain among short bubbles pay the write-back,
+        // scatter, pressure gather and set-up nearly every iteration
+        // (measured, as ms per step: 2048^2 16.9 -> 16.0, 4096^2 clustered
+        // 46.1 -> 29.2 with a batch of 24; 16-32 are equal within 2 %).
         bool fin, capped;
         double remain;
         while (true) {
@@ -479,7 +489,9 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                 }
             }
             fin = active && (!(remain > 0.0) || capped);
-            if (__ballot(fin) != 0ull) break;
+            const unsigned long long fm = __ballot(fin);
+            if (fm != 0ull && (__popcll(fm) >= kRefillBatch || __ballot(active && !fin) == 0ull))
+                break;
         }
         // ---- write back the bubbles that reached the sync time
         if (active) {
