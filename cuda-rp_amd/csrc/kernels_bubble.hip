@@ -665,7 +665,8 @@ void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev
     // lane still takes its SIMD's issue slots, so every dedicated wavefront
     // costs a SIMD of throughput there.  4096^2, 2^22 clustered bubbles
     // (full grid): 512 dedicated 49.8, 128 46.8, 32 46.4, 0 46.2 ms/step;
-    // 2048^2 (lean grid): 256 17.2, 32 17.4, 0 19.1 ms/step.
+    // with refills batched by kRefillBatch: 128 29.6, 64 29.2, 32 29.3,
+    // 0 30.8; 2048^2 (lean grid): 256 17.2, 32 17.4, 0 19.1 ms/step.
     const int solo_cap = lean ? (int)grid.x : std::max(1, g_cus / 8);
     if (host.bp.rp_exact)
         hipLaunchKernelGGL((k_bubble_persistent<F, false>), grid, dim3(256), 0, s, dev, cap,
