@@ -44,7 +44,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def c2_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
+def focused_2048_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
     cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=1.5e6, freq=1.0e6, focal_dist=15e-3,
                            bubbles=True, fg0=1e-6, precision=precision)
     r = np.random.default_rng(seed)
@@ -57,7 +57,7 @@ def c2_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
     return cfg, (x, y, R0, None, fg_in)
 
 
-def c3_workload(abi, precision, n=4096, nb=1 << 22, seed=3):
+def clustered_4096_workload(abi, precision, n=4096, nb=1 << 22, seed=3):
     """4096^2, 32-cell PML, 2 MHz 3 MPa focused CW, 2^22 log-normal bubbles
     (median 3 um, sigma_log 0.4) clustered in 16 Gaussian clouds (centres
     uniform over an 8 x 8 mm box at the focus, sigma 0.25 mm, clipped to the
@@ -77,7 +77,7 @@ def c3_workload(abi, precision, n=4096, nb=1 << 22, seed=3):
     return cfg, (x, y, R0, None, fg_in)
 
 
-def c4_workload(abi, precision, n=1024, nb=65536, n_elem=64, seed=4):
+def adjoint_1024_workload(abi, precision, n=1024, nb=65536, n_elem=64, seed=4):
     """The adjoint workload: 1024^2, 65536 bubbles with R0 ~ U(1, 5) um in a
     4 x 4 mm box at the focus (mean void fraction 1e-6 there), a 64-element
     aperture with amplitudes ~ U(0.3, 0.7) MPa and phases ~ U(0, 2 pi)."""
@@ -101,7 +101,7 @@ def c4_workload(abi, precision, n=1024, nb=65536, n_elem=64, seed=4):
 def adjoint_leg(brp, abi, steps, window, every):
     """brp_adjoint on the adjoint workload next to a plain forward of the
     same steps: adjoint/forward time."""
-    cfg, bub, src, box = c4_workload(abi, abi.PREC_MIXED)
+    cfg, bub, src, box = adjoint_1024_workload(abi, abi.PREC_MIXED)
     f = brp.Simulation(cfg)
     f.set_bubbles(*bub)
     f.set_source(*src)
@@ -225,7 +225,7 @@ def thermal_leg(brp, abi, n, nb, bub, warmup, steps, other_without_ms):
     increment T, f_g, f_g(n), heat (8 each), p.x, p.y, pn.x, pn.y (4 each in
     mixed mode), dT written (8); add T, dT read, T written (24); argmax dT
     read (8): 88 bytes."""
-    cfg, _ = c2_workload(abi, abi.PREC_MIXED, n, nb)
+    cfg, _ = focused_2048_workload(abi, abi.PREC_MIXED, n, nb)
     cfg.thermal = 1
     sim, st, wall, ph, _, _ = run_gpu(brp, cfg, bub, warmup, steps, 0)
     sim.close()
@@ -242,11 +242,11 @@ def thermal_leg(brp, abi, n, nb, bub, warmup, steps, other_without_ms):
                          "bytes_per_cell": per_cell}}
 
 
-def c3_leg(brp, abi, warmup, steps):
+def clustered_leg(brp, abi, warmup, steps):
     """The 4096^2 workload with 2^22 clustered bubbles at 2 MHz / 3 MPa
-    (c3_workload), mixed precision, timed over `steps` steps after `warmup`
+    (clustered_4096_workload), mixed precision, timed over `steps` steps after `warmup`
     (the wavefront reaches the clouds near step 200)."""
-    cfg, bub = c3_workload(abi, abi.PREC_MIXED)
+    cfg, bub = clustered_4096_workload(abi, abi.PREC_MIXED)
     n, nb = cfg.grid_x, len(bub[0])
     sim, st, wall, ph, _, _ = run_gpu(brp, cfg, bub, warmup, steps, 0)
     sim.close()
@@ -292,9 +292,9 @@ def main():
     ap.add_argument("--bubbles", type=int, default=1 << 20)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-thermal", action="store_true")
-    ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--c3-steps", type=int, default=100)
-    ap.add_argument("--c3-warmup", type=int, default=300)
+    ap.add_argument("--no-clustered", action="store_true")
+    ap.add_argument("--clustered-steps", type=int, default=100)
+    ap.add_argument("--clustered-warmup", type=int, default=300)
     ap.add_argument("--thermal-steps", type=int, default=100)
     ap.add_argument("--adjoint-steps", type=int, default=4000)
     ap.add_argument("--adjoint-window", type=int, default=1000)
@@ -321,7 +321,7 @@ def main():
     n, nb = args.grid, args.bubbles
     first = min(args.cpu_steps, args.warmup)
 
-    cfg_m, bub = c2_workload(abi, abi.PREC_MIXED, n, nb)
+    cfg_m, bub = focused_2048_workload(abi, abi.PREC_MIXED, n, nb)
     log(f"mixed: {n}^2 cells, {nb} bubbles, warmup {args.warmup}, steps {args.steps}")
     sim_m, st_m, wall_m, ph_m, tfirst_m, warm_m = run_gpu(brp, cfg_m, bub, args.warmup,
                                                           args.steps, first)
@@ -375,7 +375,7 @@ def main():
     }
 
     if not args.no_fp64:
-        cfg_d, _ = c2_workload(abi, abi.PREC_FP64, n, nb)
+        cfg_d, _ = focused_2048_workload(abi, abi.PREC_FP64, n, nb)
         log("fp64 run")
         sim_d, st_d, wall_d, ph_d, _, warm_d = run_gpu(brp, cfg_d, bub, args.warmup, args.steps,
                                                        0)
@@ -400,10 +400,10 @@ def main():
         sim_d.close()
     sim_m.close()
 
-    if not args.no_c3:
-        log("C3 leg")
-        out["c3"] = c3_leg(brp, abi, args.c3_warmup, args.c3_steps)
-        log(f"  C3: {out['c3']['ms_per_step']:.2f} ms/step")
+    if not args.no_clustered:
+        log("clustered 4096^2 leg")
+        out["clustered_4096"] = clustered_leg(brp, abi, args.clustered_warmup, args.clustered_steps)
+        log(f"  clustered: {out['clustered_4096']['ms_per_step']:.2f} ms/step")
 
     if not args.no_thermal:
         log("thermal leg")
@@ -418,7 +418,7 @@ def main():
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cfg_c, _ = c2_workload(abi, abi.PREC_FP64, n, nb)
+        cfg_c, _ = focused_2048_workload(abi, abi.PREC_FP64, n, nb)
         log(f"cpu baseline: oracle, {threads} threads, steps 1-{first}")
         wall_c, st_c = cpu_baseline(abi, cfg_c, bub, first, threads)
         out["cpu_baseline"] = {
