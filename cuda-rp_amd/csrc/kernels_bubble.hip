@@ -308,8 +308,9 @@ __device__ inline unsigned long long lanemask_lt(int lane) {
 // differs by orders of magnitude between bubbles (about 11 at rest, thousands
 // through an inertial collapse), so one bubble per lane would run each
 // wavefront at the pace of its slowest lane.  Here every lane integrates one
-// sub-step per loop pass; a lane whose bubble reaches the sync time writes it
-// back and takes the next bubble at once.  Bubbles are handed to wavefronts in
+// sub-step per loop pass; lanes whose bubbles reached the sync time are
+// written back and refilled with the next bubbles once kRefillBatch of them
+// have finished.  Bubbles are handed to wavefronts in
 // batches of 64 consecutive (cell-sorted) indices through one atomic counter
 // per Picard iteration, so a wavefront costs the sum, not the max, of its
 // lanes' sub-steps.  Per bubble the arithmetic is exactly that of the
