@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     WaveAgg &agg = agg_all[threadIdx.x >> 6];
     agg_clear(agg, lane);
     // The shared RP constants go through LDS into vector registers, held for
-    // the whole kernel (245 VGPRs, two wavefronts per SIMD still fit): read
+    // the whole kernel (247 VGPRs, two wavefronts per SIMD still fit): read
     // from the constant table instead, each sub-step waited on its scalar
     // loads (no scalar registers are free to hold them).
     __shared__ double shc[16];
@@ -372,9 +372,9 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     // The E longest bubbles (the queue's first entries) run one per workgroup
     // in lane 0 of its first wavefront, alone: a wavefront issues the same
     // instructions whether one lane or 64 are active, so a lone lane skips
-    // the paths the other lanes would need (the coth of the thermal
-    // impedance is one multiply for s > 20), and it never waits for the
-    // other lanes' refills.  The wavefront runs at raised issue priority
+    // the paths the other lanes would need (the thermal impedance is a
+    // closed form instead of a table lookup for s > 20), and it never waits
+    // for the other lanes' refills.  The wavefront runs at raised issue priority
     // until that bubble is done, then joins the shared queue, which starts
     // after the E dedicated entries.
     int E, nb;
