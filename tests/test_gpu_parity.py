@@ -205,6 +205,28 @@ def test_clustered_and_lattice_scatter(brp):
     np.testing.assert_array_equal(g.field("p0"), g2.field("p0"))
 
 
+def test_many_wavefront_schedule_bit_exact(brp):
+    """65 536 bubbles through an inertial collapse: the RP launch runs on a
+    whole-device grid with dedicated wavefronts for the longest chains,
+    batched refills and both grid shapes as the sub-step counts change, and
+    the queue order changes every launch.  The results must not depend on
+    that schedule: two runs agree bit for bit (fields, radii, sub-step
+    counts, Picard counts)."""
+    cfg, b = cases.bubbly_case(64, nb=65536, seed=7)
+    runs = []
+    for _ in range(2):
+        g = cases.make(brp.Simulation, cfg, b)
+        st = g.step(120)
+        runs.append((st, g.field("p0"), g.field("fg"), g.bubbles("R_t"), g.bubbles("sub")))
+        g.close()
+    (s1, *a1), (s2, *a2) = runs
+    print("max sub-steps", max(s["rp_max_substeps"] for s in s1), flush=True)
+    assert max(s["rp_max_substeps"] for s in s1) > 512  # long chains, dedicated wavefronts
+    assert [s["n_pic"] for s in s1] == [s["n_pic"] for s in s2]
+    for x, y in zip(a1, a2):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_wide_band_and_edge_bubbles(brp):
     """smooth_delta_band = 4, bubbles straddling the domain edges and outside it."""
     cfg, _ = cases.bubbly_case(48, nb=10)
