@@ -205,6 +205,21 @@ def test_clustered_and_lattice_scatter(brp):
     np.testing.assert_array_equal(g.field("p0"), g2.field("p0"))
 
 
+def test_cavitating_4096_bubbles_multi_workgroup(brp):
+    """4096 cavitating bubbles (16 RP workgroups, up to 16 dedicated
+    wavefronts, batched refills across many wavefronts) against the oracle,
+    FP64, 120 steps."""
+    cfg, b = cases.bubbly_case(64, nb=4096, seed=7)
+    g, o, gs, os_ = run_pair(brp, cfg, b, steps=120)
+    err = report("bubbly 4096", g, o, FIELDS + ("fg",))
+    rt = rel_l2(g.bubbles("R_t"), o.bubbles("R_t"))
+    print("R_t relL2", rt, "max sub-steps", max(s["rp_max_substeps"] for s in os_), flush=True)
+    assert max(s["rp_max_substeps"] for s in os_) > 512
+    assert [s["n_pic"] for s in gs] == [s["n_pic"] for s in os_]
+    assert err["p0"] < 1e-9 and err["fg"] < 1e-9 and rt < 1e-9
+    assert np.mean(g.bubbles("sub") != o.bubbles("sub")) <= 0.05
+
+
 def test_many_wavefront_schedule_bit_exact(brp):
     """65 536 bubbles through an inertial collapse: the RP launch runs on a
     whole-device grid with dedicated wavefronts for the longest chains,
