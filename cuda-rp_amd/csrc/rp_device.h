@@ -218,7 +218,7 @@ struct RPFastConst {   // per bubble
 struct RPFastShared {  // shared by all bubbles
     double PL0, gam, mu, rho, irho, dt0, g3, k3, C0, two_sig, four_mu, mu2, m2sig_rho,
         four_pi_K0, sixteen_pi_mu;
-    const double *vtab;  // kVTable staged in LDS by the kernel (upsilon_g, lp_g)
+    const double *vtab;  // kVTable staged in LDS by the kernel (upsilon_g, lp_inv_g)
 };
 
 }  // namespace brp
