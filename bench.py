@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the coupled FDTD + Rayleigh-Plesset step on one MI355X.
 
-Workload (one "step" = one coupled mixture time step of the whole problem):
-  2048 x 2048 cells, dx = dy = 0.1 mm, 32-cell PML on all sides, water
+Headline workload (one "step" = one coupled mixture time step of the whole
+problem), the clustered 4096^2 configuration:
+  4096 x 4096 cells, dx = dy = 0.1 mm, 32-cell PML on all sides, water
   (c = 1500 m/s, rho = 1000 kg/m^3, mu = 1e-3 Pa s, sigma = 0.0725 N/m,
-  P0 = 101325 Pa, gamma = 1.4), focused 1 MHz continuous wave of 1.5 MPa from
-  the y0 edge (focal length 15 mm, focus centred in x), 2^20 bubbles with
-  log-normal R0 (median 3 um, sigma_log 0.4) uniform in an 8 x 8 mm box around
-  the focus (seed 2), initial mean void fraction 1e-6 in the box, CFL 0.5.
+  P0 = 101325 Pa, gamma = 1.4), a focused 2 MHz continuous wave of 3 MPa from
+  the y0 edge (focal length 15 mm, focus centred in x), 2^22 bubbles with
+  log-normal R0 (median 3 um, sigma_log 0.4) in 16 seeded Gaussian clouds
+  (sigma 0.25 mm) in an 8 x 8 mm box at the focus, mean void fraction 1e-6
+  over the box, CFL 0.5, mixed precision (FP32 fields, FP64 RP and
+  void-fraction accumulation).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-W untimed steps bring the wavefront to the bubbles (the focus is ~300 steps
-from the source), then K steps are timed with the mixed-precision path
-(FP32 fields, FP64 RP and void fraction), then the same W + K steps run with
-the all-FP64 path for the speed and error comparison.  One JSON line on
-stdout.  Progress lines go to stderr.
+Every run first takes PREROLL = 300 untimed steps, which bring the wavefront
+onto the bubble clouds (the focus is ~260 steps from the source) so that the
+timed window is inside the cavitating regime the benchmark is about; then W
+further untimed warm-up steps, then K timed steps.  Beside the headline: the
+all-FP64 run of the same window, mixed-vs-FP64 window statistics, the error
+of the mixed run against the FP64 oracle seeded with the GPU's state at the
+window start, the 2048^2 workload, the temperature solve, the adjoint
+workload and the CPU baseline.  One JSON line on stdout; progress on stderr.
 """
 import argparse
 import json
@@ -30,7 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
+import cases  # noqa: E402  (workload definitions shared with the tests)
+
 METRIC = "cell-updates/s & bubble-steps/s: mixed vs FP64 at stated err; adjoint/fwd time"
+PREROLL = 300
 # Algorithmic FP64 operations of one RP sub-step (add, sub, mul, div, sqrt,
 # tanh, tan each counted once; the oracle's operation counter gives 470.6 on a
 # driven log-normal population) and of one bubble's gather + scatter.
@@ -38,111 +47,20 @@ RP_FLOPS_PER_SUBSTEP = 470.0
 BUBBLE_FLOPS_PER_CALL = 60.0
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes of one step: per cell-update 116 + 56 n_pic (mixed) or
+# 176 + 88 n_pic (FP64); per bubble-step 96 + 288 n_pic.
+CELL_BYTES = {"mixed": (116.0, 56.0), "fp64": (176.0, 88.0)}
+BUBBLE_BYTES = (96.0, 288.0)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def focused_2048_workload(abi, precision, n=2048, nb=1 << 20, seed=2):
-    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=1.5e6, freq=1.0e6, focal_dist=15e-3,
-                           bubbles=True, fg0=1e-6, precision=precision)
-    r = np.random.default_rng(seed)
-    fpx, fpy = cfg.ext_focus_x, 15e-3 * 0.5 * math.sqrt(3.0)
-    box = 8e-3
-    x = fpx + (r.random(nb) - 0.5) * box
-    y = fpy + (r.random(nb) - 0.5) * box
-    R0 = np.exp(math.log(3e-6) + 0.4 * r.standard_normal(nb))
-    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
-    return cfg, (x, y, R0, None, fg_in)
-
-
-def clustered_4096_workload(abi, precision, n=4096, nb=1 << 22, seed=3):
-    """4096^2, 32-cell PML, 2 MHz 3 MPa focused CW, 2^22 log-normal bubbles
-    (median 3 um, sigma_log 0.4) clustered in 16 Gaussian clouds (centres
-    uniform over an 8 x 8 mm box at the focus, sigma 0.25 mm, clipped to the
-    box), mean void fraction 1e-6 over the box."""
-    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=3.0e6, freq=2.0e6, focal_dist=15e-3,
-                           bubbles=True, fg0=1e-6, precision=precision)
-    r = np.random.default_rng(seed)
-    fpx, fpy = cfg.ext_focus_x, 15e-3 * 0.5 * math.sqrt(3.0)
-    box = 8e-3
-    cx = fpx + (r.random(16) - 0.5) * box
-    cy = fpy + (r.random(16) - 0.5) * box
-    k = r.integers(0, 16, nb)
-    x = np.clip(cx[k] + 0.25e-3 * r.standard_normal(nb), fpx - 0.5 * box, fpx + 0.5 * box)
-    y = np.clip(cy[k] + 0.25e-3 * r.standard_normal(nb), fpy - 0.5 * box, fpy + 0.5 * box)
-    R0 = np.exp(math.log(3e-6) + 0.4 * r.standard_normal(nb))
-    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
-    return cfg, (x, y, R0, None, fg_in)
-
-
-def adjoint_1024_workload(abi, precision, n=1024, nb=65536, n_elem=64, seed=4):
-    """The adjoint workload: 1024^2, 65536 bubbles with R0 ~ U(1, 5) um in a
-    4 x 4 mm box at the focus (mean void fraction 1e-6 there), a 64-element
-    aperture with amplitudes ~ U(0.3, 0.7) MPa and phases ~ U(0, 2 pi)."""
-    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=0.5e6, freq=1.0e6, focal_dist=15e-3,
-                           bubbles=True, fg0=1e-6, precision=precision)
-    r = np.random.default_rng(seed)
-    fpx, fpy = cfg.ext_focus_x, 15e-3 * 0.5 * math.sqrt(3.0)
-    box = 4e-3
-    x = fpx + (r.random(nb) - 0.5) * box
-    y = fpy + (r.random(nb) - 0.5) * box
-    R0 = 1e-6 + 4e-6 * r.random(nb)
-    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
-    amp = 0.3e6 + 0.4e6 * r.random(n_elem)
-    phase = 2.0 * math.pi * r.random(n_elem)
-    dx = cfg.grid_lx / n
-    ic, jc = int(round(fpx / dx + 0.5)), int(round(fpy / dx + 0.5))
-    focal_box = (ic - 10, ic + 9, jc - 10, jc + 9)  # 2 x 2 mm
-    return cfg, (x, y, R0, None, fg_in), (amp, phase), focal_box
-
-
-def adjoint_leg(brp, abi, steps, window, every):
-    """brp_adjoint on the adjoint workload next to a plain forward of the
-    same steps: adjoint/forward time."""
-    cfg, bub, src, box = adjoint_1024_workload(abi, abi.PREC_MIXED)
-    f = brp.Simulation(cfg)
-    f.set_bubbles(*bub)
-    f.set_source(*src)
-    f.prepare()
-    t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        m = min(500, steps - done)
-        f.step(m)
-        done += m
-        log(f"  adjoint workload forward step {done}: {(time.perf_counter() - t0) / done * 1e3:.2f} ms/step")
-    t_fwd = time.perf_counter() - t0
-    f.close()
-    a = brp.Simulation(cfg)
-    a.set_bubbles(*bub)
-    a.set_source(*src)
-    a.prepare()
-    log(f"  adjoint: {steps} steps, window {window}, checkpoint every {every}")
-    t0 = time.perf_counter()
-    r = a.adjoint(box, steps, window, every)
-    t_adj = time.perf_counter() - t0
-    a.close()
-    g = np.concatenate([r["grad_amp"], r["grad_phase"]])
-    log(f"  adjoint: {t_adj:.2f} s vs forward {t_fwd:.2f} s")
-    return {"workload": "1024x1024 grid, 65536 bubbles R0~U(1,5) um in a 4x4 mm box, "
-                        "64 elements amp~U(0.3,0.7) MPa phase~U(0,2pi), mixed precision",
-            "steps": steps, "window": window, "checkpoint_every": every,
-            "J": r["J"], "forward_s": t_fwd, "adjoint_s": t_adj,
-            "adjoint_over_forward": t_adj / t_fwd,
-            "adjoint_forward_sweep_s": r["forward_ms"] / 1e3,
-            "adjoint_reverse_sweep_s": r["reverse_ms"] / 1e3,
-            "memory_gb": r["memory_bytes"] / 1e9, "n_checkpoints": r["n_checkpoints"],
-            "grad_norm": float(np.linalg.norm(g)),
-            "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())}
-
-
 def stepped(sim, k, tag, chunk=25):
-    """sim.step(k) in chunks, printing one progress line per chunk."""
-    out = []
-    phases = None
-    done = 0
+    """sim.step(k) in chunks, one progress line per chunk; returns the step
+    records and the summed device phase times (ms)."""
+    out, phases, done = [], None, 0
     while done < k:
         m = min(chunk, k - done)
         t0 = time.perf_counter()
@@ -159,54 +77,75 @@ def stepped(sim, k, tag, chunk=25):
     return out, phases
 
 
-def run_gpu(brp, cfg, bub, warmup, steps, first):
+def timed_window(brp, cfg, bub, warmup, steps, tag, keep_state=False):
+    """PREROLL + warmup untimed steps, then `steps` timed ones (synchronous:
+    each brp_step call returns after its last kernel).  With keep_state the
+    state at the window start is saved in slot 0 and exported."""
     sim = brp.Simulation(cfg)
     sim.set_bubbles(*bub)
     sim.prepare()
+    stepped(sim, PREROLL + warmup, f"{tag} pre-roll", chunk=50)
+    state = None
+    if keep_state:
+        sim.save_state(0)
+        state = sim.export_state()
     t0 = time.perf_counter()
-    st_first = sim.step(first) if first else []
-    t_first = time.perf_counter() - t0
-    if warmup > first:
-        stepped(sim, warmup - first, "warmup", chunk=50)
-    at_warmup = (sim.field("p0"), sim.bubbles("R_t"))
-    log(f"  warmup done ({warmup} steps), timing {steps} steps")
-    t0 = time.perf_counter()
-    st, phases = stepped(sim, steps, "timed")
+    st, ph = stepped(sim, steps, f"{tag} timed")
     wall = time.perf_counter() - t0
-    return sim, st, wall, phases, t_first, at_warmup
+    return sim, st, wall, ph, state
 
 
-def cpu_baseline(abi, cfg, bub, sample_steps, threads):
-    import oracle_lib as ol
-    o = ol.OracleSim(cfg, threads=threads)
-    o.set_bubbles(*bub)
-    o.prepare()
-    t0 = time.perf_counter()
-    st = o.step(sample_steps)
-    wall = time.perf_counter() - t0
-    o.close()
-    return wall, st
+def summarize(st, wall, ph, n, nb, prec):
+    steps = len(st)
+    launches = sum(s["n_pic"] for s in st)
+    substeps = sum(s["rp_mean_substeps"] * s["n_pic"] * nb for s in st)
+    a, b = CELL_BYTES[prec]
+    step_bytes = sum(n * n * (a + b * s["n_pic"]) + nb * (BUBBLE_BYTES[0] + BUBBLE_BYTES[1] *
+                                                          s["n_pic"]) for s in st)
+    return {"ms_per_step": wall / steps * 1e3, "cell_updates_per_s": n * n * steps / wall,
+            "bubble_steps_per_s": nb * steps / wall, "rp_substeps_per_s": substeps / wall,
+            "picard_per_step": launches / steps, "launches": launches, "substeps": substeps,
+            "rp_launch_ms": ph["bubble_rp"] / max(launches, 1),
+            "rp_max_substeps": max(s["rp_max_substeps"] for s in st),
+            "rp_lean_launches": sum(s["rp_lean_launches"] for s in st),
+            "phase_ms_per_step": {k: v / steps for k, v in ph.items()},
+            "device_ms_per_step": sum(ph.values()) / steps,
+            "step_bytes": step_bytes / steps}
 
 
-def measured_traffic():
-    """HBM bytes per RP launch (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
-    from the committed counter summary, or None."""
-    path = os.path.join(ROOT, "profiles", "k_bubble_persistent_2048_counters.md")
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
+def rp_roofline(sm, nb, traffic):
+    """The dominant kernel, k_bubble_persistent (gather + RP + scatter of every
+    bubble, one launch per Picard iteration): algorithmic FP64 operations per
+    launch over its mean launch time from HIP events on the library's stream."""
+    flops = (RP_FLOPS_PER_SUBSTEP * sm["substeps"] + BUBBLE_FLOPS_PER_CALL * nb * sm["launches"]) \
+        / max(sm["launches"], 1)
+    achieved = flops / (sm["rp_launch_ms"] * 1e-3) / 1e12
+    return {"kernel": "k_bubble_persistent (pressure gather + Rayleigh-Plesset + void-fraction "
+                      "scatter, one launch per Picard iteration)",
+            "bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "avg_launch_ms": sm["rp_launch_ms"], "flops_per_launch": flops,
+            "substeps_per_launch": sm["substeps"] / max(sm["launches"], 1),
+            "note": "FP64 vector ALU (no MFMA: the RP sub-step is a scalar recurrence per "
+                    "bubble); peak = MI355X FP64 vector 78.6 TFLOP/s.  traffic = HBM bytes "
+                    "per launch, FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 (copy-calibrated)"}
+
+
+def step_roofline(sm, prec):
+    achieved = sm["step_bytes"] / (sm["device_ms_per_step"] * 1e-3) / 1e9
+    a, b = CELL_BYTES[prec]
+    return {"scope": "whole coupled step (all kernels), device time",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "bytes_per_step": sm["step_bytes"],
+            "model": f"N^2 ({a:.0f} + {b:.0f} n_pic) + Nb (96 + 288 n_pic) bytes per step"}
 
 
 def grid_roofline(ph, n, steps, mixed):
     """The grid kernels (velocity, first pressure evaluation and their ghost
     kernels) against HBM: 20 + 32 algorithmic bytes per cell-update in mixed
-    mode (vx, vy, p0 read, vx, vy written; vx, vy, pn.x, pn.y, p0 read, p.x,
-    p.y, p0 written), twice that in FP64; the Picard re-evaluations cover the
-    bubble window only and are left out of the byte count."""
+    mode, twice that in FP64; the window-only Picard re-evaluations are left
+    out of the byte count."""
     per_cell = 52.0 if mixed else 104.0
     ms = (ph["velocity"] + ph["pressure"]) / steps
     achieved = per_cell * n * n / (ms * 1e-3) / 1e9
@@ -216,51 +155,198 @@ def grid_roofline(ph, n, steps, mixed):
             "bytes_per_cell_update": per_cell}
 
 
-def thermal_leg(brp, abi, n, nb, bub, warmup, steps, other_without_ms):
-    """The same workload with the one-way temperature solve and the focal
-    search on (bubble heat scatter, conductivity/flux/increment pass, the add
-    pass, ghosts, argmax of the work field, one 16-byte read-back), timed over
-    `steps` steps after `warmup`.  Its device time is the "other" phase minus
-    the same phase without the thermal solve.  Algorithmic bytes per cell:
-    increment T, f_g, f_g(n), heat (8 each), p.x, p.y, pn.x, pn.y (4 each in
-    mixed mode), dT written (8); add T, dT read, T written (24); argmax dT
-    read (8): 88 bytes."""
-    cfg, _ = focused_2048_workload(abi, abi.PREC_MIXED, n, nb)
-    cfg.thermal = 1
-    sim, st, wall, ph, _, _ = run_gpu(brp, cfg, bub, warmup, steps, 0)
-    sim.close()
-    th_ms = max(ph["other"] / steps - other_without_ms, 1e-9)
-    per_cell = 88.0
-    achieved = per_cell * n * n / (th_ms * 1e-3) / 1e9
-    return {"ms_per_step": wall / steps * 1e3, "thermal_ms_per_step": th_ms,
-            "steps_window": f"{warmup + 1}-{warmup + steps}",
-            "focal_y_last": st[-1]["focal_y"],
-            "roofline": {"kernels": "k_bubble_heat + k_temperature_increment + k_temperature_add "
-                                    "+ k_temperature_ghosts + k_focal_blocks/final",
-                         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "bytes_per_cell": per_cell}}
+def measured_traffic(name):
+    """HBM bytes per RP launch (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
+    from a committed counter summary, or None."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
-def clustered_leg(brp, abi, warmup, steps):
-    """The 4096^2 workload with 2^22 clustered bubbles at 2 MHz / 3 MPa
-    (clustered_4096_workload), mixed precision, timed over `steps` steps after `warmup`
-    (the wavefront reaches the clouds near step 200)."""
-    cfg, bub = clustered_4096_workload(abi, abi.PREC_MIXED)
+def focal_cells(cfg, half=10):
+    return cases.focal_box(cfg, half)
+
+
+def window_statistics(sim_m, sim_d, cfg, steps):
+    """Mixed against FP64 over `steps` more steps of both runs: pointwise
+    errors (which grow as collapse times drift apart) and the statistics a
+    user of the simulation reads -- the time-averaged focal pressure squared
+    (the heat-deposition proxy), the R_t / R0 percentiles and the fraction of
+    bubbles expanded beyond 2 R0."""
+    i0, i1, j0, j1 = focal_cells(cfg)
+    acc = {"m": 0.0, "d": 0.0}
+    for _ in range(steps):
+        for tag, s in (("m", sim_m), ("d", sim_d)):
+            s.step(1)
+            p = s.field("p0")[j0:j1 + 1, i0:i1 + 1]
+            acc[tag] += float(np.mean(p * p)) / steps
+    R0 = sim_m.bubbles("R0")
+    rm, rd = sim_m.bubbles("R_t") / R0, sim_d.bubbles("R_t") / R0
+    pct = (50, 90, 99, 99.9)
+    qm, qd = np.percentile(rm, pct), np.percentile(rd, pct)
+    return {"steps": steps,
+            "focal_p2_mean_mixed": acc["m"], "focal_p2_mean_fp64": acc["d"],
+            "focal_p2_rel_err": abs(acc["m"] - acc["d"]) / acc["d"],
+            "R_over_R0_percentiles": {f"p{p}": [float(a), float(b)] for p, a, b in zip(pct, qm, qd)},
+            "R_over_R0_percentile_rel_err_max": float(np.max(np.abs(qm - qd) / qd)),
+            "expanded_2R0_fraction": [float(np.mean(rm > 2.0)), float(np.mean(rd > 2.0))],
+            "p0_rel_l2_pointwise": cases.rel_l2(sim_m.field("p0"), sim_d.field("p0")),
+            "R_t_rel_l2_pointwise": cases.rel_l2(sim_m.bubbles("R_t"), sim_d.bubbles("R_t"))}
+
+
+def oracle_threads():
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env or len(os.sched_getaffinity(0))
+
+
+def oracle_seeded(cfg, bub, state, steps, threads):
+    """The FP64 oracle moved to a GPU state and advanced `steps` steps."""
+    import oracle_lib as ol
+    ocfg = cases.abi.Config.from_buffer_copy(cfg)
+    ocfg.precision = cases.abi.PREC_FP64
+    o = ol.OracleSim(ocfg, threads=threads)
+    o.set_bubbles(*bub)
+    o.prepare()
+    o.import_state(state)
+    t0 = time.perf_counter()
+    st = o.step(steps)
+    return o, st, time.perf_counter() - t0
+
+
+def oracle_leg(brp, sim_m, cfg, bub, state, steps, threads):
+    """Error of the mixed GPU run against the FP64 oracle, both started from
+    the GPU's state at the window start (slot 0) and advanced `steps` steps;
+    the oracle's time is the CPU baseline."""
     n, nb = cfg.grid_x, len(bub[0])
-    sim, st, wall, ph, _, _ = run_gpu(brp, cfg, bub, warmup, steps, 0)
-    sim.close()
-    launches = sum(s["n_pic"] for s in st)
-    return {"workload": f"{n}x{n} grid, 32-cell PML, 2 MHz 3 MPa focused CW, {nb} log-normal "
-                        "bubbles in 16 Gaussian clouds (sigma 0.25 mm) in an 8x8 mm box, mixed "
-                        "precision",
-            "steps_window": f"{warmup + 1}-{warmup + steps}",
-            "ms_per_step": wall / steps * 1e3, "value": n * n * steps / wall,
-            "unit": "cell-updates/s", "bubble_steps_per_s": nb * steps / wall,
-            "picard_per_step": launches / steps,
-            "rp_launch_ms": ph["bubble_rp"] / max(launches, 1),
-            "rp_max_substeps": max(s["rp_max_substeps"] for s in st),
-            "phase_ms_per_step": {k: v / steps for k, v in ph.items()}}
+    log(f"oracle: seeded at the window start, {steps} steps on {threads} threads")
+    o, ost, wall = oracle_seeded(cfg, bub, state, steps, threads)
+    log(f"  oracle: {wall:.1f} s")
+    sim_m.restore_state(0)
+    gst = sim_m.step(steps)
+    subset = cases.rng(6).choice(nb, min(4096, nb), replace=False)
+    err = {"steps": steps,
+           "p0_rel_l2": cases.rel_l2(sim_m.field("p0"), o.field("p0")),
+           "p0_rel_linf": cases.rel_linf(sim_m.field("p0"), o.field("p0")),
+           "vx_rel_l2": cases.rel_l2(sim_m.field("vx"), o.field("vx")),
+           "fg_rel_l2": cases.rel_l2(sim_m.field("fg"), o.field("fg")),
+           "R_t_rel_l2_subset4096": cases.rel_l2(sim_m.bubbles("R_t")[subset],
+                                                 o.bubbles("R_t")[subset]),
+           "picard_gpu": [s["n_pic"] for s in gst], "picard_oracle": [s["n_pic"] for s in ost]}
+    o.close()
+    base = {"value": n * n * steps / wall, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"the FP64 oracle (g++ -O2, OpenMP) seeded with the GPU state at the start "
+                      f"of the timed window (cavitating), {steps} step(s), {wall:.1f} s",
+            "bubble_steps_per_s": nb * steps / wall,
+            "nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    return err, base
+
+
+def single_thread_baseline(brp):
+    """The oracle on one core on the C1 workload (256^2, 1024 bubbles) over
+    steps 301-340 (cavitating), beside the GPU (FP64) on the same steps."""
+    import oracle_lib as ol
+    cfg, bub, src = cases.c1_case()
+    out = {}
+    for tag, mk in (("cpu", lambda c: ol.OracleSim(c, threads=1)), ("gpu", brp.Simulation)):
+        s = cases.make(mk, cfg, bub, src)
+        s.step(300)
+        t0 = time.perf_counter()
+        s.step(40)
+        out[tag] = time.perf_counter() - t0
+        s.close()
+    n = cfg.grid_x
+    return {"value": n * n * 40 / out["cpu"], "unit": "cell-updates/s", "cores": 1,
+            "sample": f"C1 workload (256^2, 1024 bubbles, 0.5 MPa), steps 301-340, "
+                      f"{out['cpu']:.1f} s", "gpu_same_sample_value": n * n * 40 / out["gpu"]}
+
+
+def focused_2048_leg(brp, warmup, steps):
+    """The 2048^2 workload (2^20 bubbles, 1.5 MPa), mixed and FP64, same
+    window after the pre-roll."""
+    out = {}
+    for prec, name in ((1, "mixed"), (0, "fp64")):
+        cfg, bub = cases.c2_case(precision=prec)
+        sim, st, wall, ph, _ = timed_window(brp, cfg, bub, warmup, steps, f"2048 {name}")
+        sm = summarize(st, wall, ph, cfg.grid_x, len(bub[0]), name)
+        sm["grid_roofline"] = grid_roofline(ph, cfg.grid_x, steps, prec == 1)
+        out[name] = sm
+        out[f"_{name}_sim"] = sim
+    e = {"p0_rel_l2": cases.rel_l2(out["_mixed_sim"].field("p0"), out["_fp64_sim"].field("p0")),
+         "R_t_rel_l2": cases.rel_l2(out["_mixed_sim"].bubbles("R_t"),
+                                    out["_fp64_sim"].bubbles("R_t"))}
+    for k in ("_mixed_sim", "_fp64_sim"):
+        out.pop(k).close()
+    return {"workload": "2048x2048 grid, 32-cell PML, 1 MHz 1.5 MPa focused CW, 1048576 "
+                        "log-normal bubbles in an 8x8 mm box",
+            "steps_window": f"{PREROLL + warmup + 1}-{PREROLL + warmup + steps}",
+            "mixed": out["mixed"], "fp64": out["fp64"], "mixed_vs_fp64_at_window_end": e}
+
+
+def thermal_leg(brp, warmup, steps):
+    """The 2048^2 workload with the one-way temperature solve and focal search
+    on, beside the same run without it: the difference of their "other"
+    phase is the temperature solve.  Algorithmic bytes per cell: increment
+    (T, f_g, f_g(n), heat 8 each; p.x, p.y, pn.x, pn.y 4 each; dT written 8),
+    add (T, dT read, T written: 24), argmax (dT read: 8): 88 bytes."""
+    res = {}
+    for th in (0, 1):
+        cfg, bub = cases.c2_case(precision=1)
+        cfg.thermal = th
+        sim, st, wall, ph, _ = timed_window(brp, cfg, bub, warmup, steps, f"thermal={th}")
+        res[th] = (st, wall, ph)
+        sim.close()
+    n = 2048
+    th_ms = max((res[1][2]["other"] - res[0][2]["other"]) / steps, 1e-9)
+    achieved = 88.0 * n * n / (th_ms * 1e-3) / 1e9
+    return {"ms_per_step": res[1][1] / steps * 1e3, "thermal_ms_per_step": th_ms,
+            "focal_y_last": res[1][0][-1]["focal_y"],
+            "roofline": {"kernels": "k_bubble_heat + k_temperature_increment + "
+                                    "k_temperature_add + k_temperature_ghosts + k_focal_*",
+                         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": 88.0}}
+
+
+def adjoint_leg(brp, steps, window, every):
+    """brp_adjoint on the 1024^2 adjoint workload beside a plain forward of
+    the same steps: adjoint/forward time; the library refuses (raises) when
+    its replay of the RP schedule differs from the forward's."""
+    abi = brp.abi
+    cfg, bub, src, box = cases.c4_case(precision=abi.PREC_MIXED)
+    f = cases.make(brp.Simulation, cfg, bub, src)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        m = min(500, steps - done)
+        f.step(m)
+        done += m
+        log(f"  adjoint workload forward step {done}: "
+            f"{(time.perf_counter() - t0) / done * 1e3:.2f} ms/step")
+    t_fwd = time.perf_counter() - t0
+    f.close()
+    a = cases.make(brp.Simulation, cfg, bub, src)
+    log(f"  adjoint: {steps} steps, window {window}, checkpoint every {every}")
+    t0 = time.perf_counter()
+    r = a.adjoint(box, steps, window, every)
+    t_adj = time.perf_counter() - t0
+    a.close()
+    g = np.concatenate([r["grad_amp"], r["grad_phase"]])
+    log(f"  adjoint: {t_adj:.2f} s vs forward {t_fwd:.2f} s")
+    return {"workload": "1024x1024 grid, 65536 bubbles R0~U(1,5) um in a 4x4 mm box, "
+                        "64 elements amp~U(0.3,0.7) MPa phase~U(0,2pi), mixed precision",
+            "steps": steps, "window": window, "checkpoint_every": every,
+            "J": r["J"], "forward_s": t_fwd, "adjoint_s": t_adj,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "adjoint_forward_sweep_s": r["forward_ms"] / 1e3,
+            "adjoint_reverse_sweep_s": r["reverse_ms"] / 1e3,
+            "memory_gb": r["memory_bytes"] / 1e9, "n_checkpoints": r["n_checkpoints"],
+            "schedule_mismatch": r["schedule_mismatch"],
+            "grad_norm": float(np.linalg.norm(g)),
+            "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())}
 
 
 def calibrate_copy(mib=1024, reps=5):
@@ -280,162 +366,120 @@ def calibrate_copy(mib=1024, reps=5):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline window (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain copy kernel of 1 GiB (counter calibration)")
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--grid", type=int, default=2048)
-    ap.add_argument("--bubbles", type=int, default=1 << 20)
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-thermal", action="store_true")
-    ap.add_argument("--no-clustered", action="store_true")
-    ap.add_argument("--clustered-steps", type=int, default=100)
-    ap.add_argument("--clustered-warmup", type=int, default=300)
-    ap.add_argument("--thermal-steps", type=int, default=100)
+    ap.add_argument("--stat-steps", type=int, default=20)
+    ap.add_argument("--oracle-steps", type=int, default=1)
+    ap.add_argument("--leg-steps", type=int, default=50)
     ap.add_argument("--adjoint-steps", type=int, default=4000)
     ap.add_argument("--adjoint-window", type=int, default=1000)
     ap.add_argument("--adjoint-checkpoint", type=int, default=200)
-    ap.add_argument("--adjoint-only", action="store_true",
-                    help="only the adjoint leg (printed as one JSON line)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--no-legs", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the coupled step runs on one MI355X)",
-              file=sys.stderr)
+              "(the coupled step runs on one MI355X)", file=sys.stderr)
         sys.exit(2)
 
     import importlib
     brp = importlib.import_module("cuda-rp_amd")
     abi = brp.abi
-    if args.adjoint_only:
-        print(json.dumps(adjoint_leg(brp, abi, args.adjoint_steps, args.adjoint_window,
-                                     args.adjoint_checkpoint)), flush=True)
-        return
-    n, nb = args.grid, args.bubbles
-    first = min(args.cpu_steps, args.warmup)
+    K, W = args.steps, args.warmup
 
-    cfg_m, bub = focused_2048_workload(abi, abi.PREC_MIXED, n, nb)
-    log(f"mixed: {n}^2 cells, {nb} bubbles, warmup {args.warmup}, steps {args.steps}")
-    sim_m, st_m, wall_m, ph_m, tfirst_m, warm_m = run_gpu(brp, cfg_m, bub, args.warmup,
-                                                          args.steps, first)
-    cells = n * n
-    cups = cells * args.steps / wall_m
-    bsps = nb * args.steps / wall_m
-    launches = sum(s["n_pic"] for s in st_m)
-    substeps = sum(s["rp_mean_substeps"] * s["n_pic"] * nb for s in st_m)
-    rp_ms = ph_m["bubble_rp"]
-    avg_ms = rp_ms / max(launches, 1)
-    flops_per_launch = (RP_FLOPS_PER_SUBSTEP * substeps + BUBBLE_FLOPS_PER_CALL * nb * launches) \
-        / max(launches, 1)
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    log(f"  mixed: {wall_m / args.steps * 1e3:.3f} ms/step, {cups:.3e} cell-updates/s, "
-        f"phases {ph_m}, Picard {launches / args.steps:.2f}/step")
-
+    cfg_m, bub = cases.c3_case(precision=abi.PREC_MIXED)
+    n, nb = cfg_m.grid_x, len(bub[0])
+    log(f"headline: {n}^2 cells, {nb} clustered bubbles, pre-roll {PREROLL} + warmup {W}, "
+        f"timed {K}")
+    keep = not (args.headline_only or args.no_oracle)
+    sim_m, st_m, wall_m, ph_m, state = timed_window(brp, cfg_m, bub, W, K, "mixed", keep)
+    sm = summarize(st_m, wall_m, ph_m, n, nb, "mixed")
+    log(f"  mixed: {sm['ms_per_step']:.3f} ms/step, RP launch {sm['rp_launch_ms']:.3f} ms, "
+        f"Picard {sm['picard_per_step']:.2f}/step")
+    window = f"{PREROLL + W + 1}-{PREROLL + W + K}"
     out = {
         "metric": METRIC,
-        "value": cups,
+        "value": sm["cell_updates_per_s"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall_m / args.steps * 1e3,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": sm["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 fields + f64 RP / int64 void fraction (mixed)",
-        "data": "synthetic (seeded bubble population, analytic source)",
-        "config": {"workload": f"{n}x{n} grid, 32-cell PML, 1 MHz 1.5 MPa focused CW, "
-                               f"{nb} log-normal bubbles in an 8x8 mm box, CFL 0.5",
+        "data": "synthetic (seeded clustered bubble population, analytic source)",
+        "config": {"workload": f"{n}x{n} grid, 32-cell PML, 2 MHz 3 MPa focused CW, {nb} "
+                               "log-normal bubbles in 16 Gaussian clouds (sigma 0.25 mm) in an "
+                               "8x8 mm box, CFL 0.5",
                    "grid": n, "bubbles": nb, "precision": "mixed",
-                   "steps_window": f"{args.warmup + 1}-{args.warmup + args.steps}"},
-        "bubble_steps_per_s": bsps,
-        "picard_per_step": launches / args.steps,
-        "rp_substeps_per_s": substeps / wall_m,
-        "phase_ms_per_step": {k: v / args.steps for k, v in ph_m.items()},
-        "roofline": {"kernel": "k_bubble_persistent (gather + Rayleigh-Plesset + scatter)",
-                     "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": measured_traffic(), "avg_launch_ms": avg_ms,
-                     "flops_per_launch": flops_per_launch,
-                     "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both "
-                             "78.6 TFLOP/s.  A launch is bound by the longest bubbles' "
-                             "sequential sub-step chains, each on a wavefront of its own "
-                             "that issues at most one instruction per 4 cycles whatever its "
-                             "active lanes (about 700 instructions per sub-step), not by FP64 "
-                             "throughput; traffic = HBM bytes per launch from rocprofv3 "
-                             "FETCH_SIZE / WRITE_SIZE (profiles/)"},
-        "grid_roofline": grid_roofline(ph_m, n, args.steps, mixed=True),
+                   "preroll_steps": PREROLL, "steps_window": window},
+        "bubble_steps_per_s": sm["bubble_steps_per_s"],
+        "rp_substeps_per_s": sm["rp_substeps_per_s"],
+        "picard_per_step": sm["picard_per_step"],
+        "rp_max_substeps": sm["rp_max_substeps"],
+        "rp_lean_launches": sm["rp_lean_launches"],
+        "phase_ms_per_step": sm["phase_ms_per_step"],
+        "roofline": rp_roofline(sm, nb, measured_traffic("k_bubble_persistent_4096_counters.md")),
+        "step_roofline": step_roofline(sm, "mixed"),
+        "grid_roofline": grid_roofline(ph_m, n, K, mixed=True),
     }
+    if args.headline_only:
+        print(json.dumps(out), flush=True)
+        return
 
     if not args.no_fp64:
-        cfg_d, _ = focused_2048_workload(abi, abi.PREC_FP64, n, nb)
-        log("fp64 run")
-        sim_d, st_d, wall_d, ph_d, _, warm_d = run_gpu(brp, cfg_d, bub, args.warmup, args.steps,
-                                                       0)
-        cups_d = cells * args.steps / wall_d
-        from cases import rel_l2
-        err_p0 = rel_l2(sim_m.field("p0"), sim_d.field("p0"))
-        err_rt = rel_l2(sim_m.bubbles("R_t"), sim_d.bubbles("R_t"))
-        out["fp64"] = {"value": cups_d, "ms_per_step": wall_d / args.steps * 1e3,
-                       "bubble_steps_per_s": nb * args.steps / wall_d,
-                       "phase_ms_per_step": {k: v / args.steps for k, v in ph_d.items()},
-                       "picard_per_step": sum(s["n_pic"] for s in st_d) / args.steps,
-                       "grid_roofline": grid_roofline(ph_d, n, args.steps, mixed=False)}
-        out["mixed_vs_fp64"] = {"speedup": cups / cups_d, "p0_rel_l2": err_p0,
-                                "R_t_rel_l2": err_rt,
-                                "at_step": args.warmup + args.steps,
-                                "p0_rel_l2_at_warmup_end": rel_l2(warm_m[0], warm_d[0]),
-                                "R_t_rel_l2_at_warmup_end": rel_l2(warm_m[1], warm_d[1]),
-                                "warmup_end_step": args.warmup,
-                                "picard_count_mismatch_steps":
-                                    int(sum(a["n_pic"] != b["n_pic"] for a, b in zip(st_m, st_d)))}
-        log(f"  fp64: {wall_d / args.steps * 1e3:.3f} ms/step; p0 err {err_p0:.2e}")
+        cfg_d, _ = cases.c3_case(precision=abi.PREC_FP64)
+        sim_d, st_d, wall_d, ph_d, _ = timed_window(brp, cfg_d, bub, W, K, "fp64")
+        sd = summarize(st_d, wall_d, ph_d, n, nb, "fp64")
+        out["fp64"] = {"value": sd["cell_updates_per_s"], "ms_per_step": sd["ms_per_step"],
+                       "bubble_steps_per_s": sd["bubble_steps_per_s"],
+                       "picard_per_step": sd["picard_per_step"],
+                       "rp_launch_ms": sd["rp_launch_ms"],
+                       "phase_ms_per_step": sd["phase_ms_per_step"],
+                       "step_roofline": step_roofline(sd, "fp64"),
+                       "grid_roofline": grid_roofline(ph_d, n, K, mixed=False)}
+        mvd = {"speedup": sm["cell_updates_per_s"] / sd["cell_updates_per_s"],
+               "picard_count_mismatch_steps":
+                   int(sum(a["n_pic"] != b["n_pic"] for a, b in zip(st_m, st_d))),
+               "at_window_end": {
+                   "p0_rel_l2": cases.rel_l2(sim_m.field("p0"), sim_d.field("p0")),
+                   "R_t_rel_l2": cases.rel_l2(sim_m.bubbles("R_t"), sim_d.bubbles("R_t"))}}
+        log("window statistics, mixed vs FP64")
+        mvd["window_after"] = window_statistics(sim_m, sim_d, cfg_m, args.stat_steps)
+        out["mixed_vs_fp64"] = mvd
         sim_d.close()
+
+    if keep:
+        threads = oracle_threads()
+        err, base = oracle_leg(brp, sim_m, cfg_m, bub, state, args.oracle_steps, threads)
+        out["mixed_vs_oracle"] = err
+        out["cpu_baseline"] = base
+        del state
+        log("cpu baseline: one thread, C1 workload")
+        out["cpu_baseline"]["single_thread"] = single_thread_baseline(brp)
     sim_m.close()
 
-    if not args.no_clustered:
-        log("clustered 4096^2 leg")
-        out["clustered_4096"] = clustered_leg(brp, abi, args.clustered_warmup, args.clustered_steps)
-        log(f"  clustered: {out['clustered_4096']['ms_per_step']:.2f} ms/step")
-
-    if not args.no_thermal:
+    if not args.no_legs:
+        log("2048^2 leg")
+        out["focused_2048"] = focused_2048_leg(brp, W, args.leg_steps)
         log("thermal leg")
-        out["thermal"] = thermal_leg(brp, abi, n, nb, bub, args.warmup, args.thermal_steps,
-                                     ph_m["other"] / args.steps)
-        log(f"  thermal: {out['thermal']['thermal_ms_per_step']:.3f} ms/step")
-
+        out["thermal"] = thermal_leg(brp, W, args.leg_steps)
     if not args.no_adjoint:
         log("adjoint leg")
-        out["adjoint"] = adjoint_leg(brp, abi, args.adjoint_steps, args.adjoint_window,
+        out["adjoint"] = adjoint_leg(brp, args.adjoint_steps, args.adjoint_window,
                                      args.adjoint_checkpoint)
-
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cfg_c, _ = focused_2048_workload(abi, abi.PREC_FP64, n, nb)
-        log(f"cpu baseline: oracle, {threads} threads, steps 1-{first}")
-        wall_c, st_c = cpu_baseline(abi, cfg_c, bub, first, threads)
-        out["cpu_baseline"] = {
-            "value": cells * first / wall_c, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"the FP64 oracle on this workload, steps 1-{first} from rest "
-                      f"({wall_c:.1f} s; the bubbles are still at equilibrium)",
-            "bubble_steps_per_s": nb * first / wall_c,
-            "gpu_same_sample_value": cells * first / tfirst_m,
-        }
-        if threads > 1:  # the same oracle on one core, on a shorter sample
-            k1 = max(1, min(2, first))
-            log(f"cpu baseline: oracle, 1 thread, steps 1-{k1}")
-            wall_1, _ = cpu_baseline(abi, cfg_c, bub, k1, 1)
-            out["cpu_baseline"]["single_thread"] = {
-                "value": cells * k1 / wall_1, "unit": "cell-updates/s", "cores": 1,
-                "sample": f"steps 1-{k1} ({wall_1:.1f} s)"}
     print(json.dumps(out), flush=True)
 
 

@@ -20,9 +20,7 @@ from . import abi
 from .abi import *  # noqa: F401,F403  (re-export the ABI mirror)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# BRP_LIBRARY selects another build of the same library (e.g. a variant built
-# with different compile-time options); by default the in-tree build.
-LIB_PATH = os.environ.get("BRP_LIBRARY") or os.path.join(_HERE, "lib", "libbubbles_rp.so")
+LIB_PATH = os.path.join(_HERE, "lib", "libbubbles_rp.so")
 _lib = None
 
 
@@ -60,6 +58,9 @@ def load_library():
         "brp_restore_state": (C.c_int, [vp, i32]),
         "brp_get_phase_times": (C.c_int, [vp, dp]),
         "brp_set_focus": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "brp_set_bubble_array": (C.c_int, [vp, i32, dp]),
+        "brp_get_clock": (C.c_int, [vp, C.POINTER(i64), dp, dp]),
+        "brp_set_clock": (C.c_int, [vp, i64, C.c_double, C.c_double]),
         "brp_adjoint": (C.c_int, [vp, C.POINTER(abi.Objective), dp, dp, dp, i64,
                                   C.POINTER(abi.AdjointStats)]),
     }
@@ -76,7 +77,7 @@ def _dptr(a):
 
 
 def stats_to_dicts(arr):
-    return [{k: getattr(s, k) for k, _ in abi.StepStats._fields_ if k != "pad_"} for s in arr]
+    return [{k: getattr(s, k) for k, _ in abi.StepStats._fields_} for s in arr]
 
 
 class BrpError(RuntimeError):
@@ -155,6 +156,43 @@ class _Backend:
         if n:
             self._check(self._f("get_bubbles")(self.h, abi.BUB[name], _dptr(out)))
         return out
+
+    def set_bubble_array(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self._f("set_bubble_array")(self.h, abi.BUB[name], _dptr(v)))
+
+    def clock(self):
+        n, t, c = C.c_int64(), C.c_double(), C.c_double()
+        self._check(self._f("get_clock")(self.h, C.byref(n), C.byref(t), C.byref(c)))
+        return n.value, t.value, c.value
+
+    def set_clock(self, nstep, time, carry):
+        self._check(self._f("set_clock")(self.h, nstep, time, carry))
+
+    # The state one step starts from: the fields the step reads and the bubble
+    # arrays of bubble_t it carries over (positions, RP state, pressure
+    # history, outputs), plus the clock.
+    STATE_FIELDS = ("p0", "px", "py", "vx", "vy", "fg", "fgn", "fgm", "rho_m")
+    STATE_BUBBLES = ("x", "y", "R_t", "R_p", "R_n", "d1R", "PG", "PL_p", "PL_n", "PL_m",
+                     "Q_B", "dt", "re", "vx", "vy")
+
+    def export_state(self):
+        return {"fields": {n: self.field(n) for n in self.STATE_FIELDS},
+                "bubbles": {n: self.bubbles(n) for n in self.STATE_BUBBLES}
+                if self.info()["n_bubbles"] else {},
+                "clock": self.clock()}
+
+    def import_state(self, st):
+        """Moves this run to a state exported by another run of the same
+        configuration and bubble population (f_g first: setting it sets f_gn
+        and f_gm too)."""
+        for n in self.STATE_FIELDS:
+            if n == "rho_m" and self.prefix == "brp_":
+                continue  # the library derives rho_m from f_g
+            self.set_field(n, st["fields"][n])
+        for n, v in st["bubbles"].items():
+            self.set_bubble_array(n, v)
+        self.set_clock(*st["clock"])
 
     def __enter__(self):
         return self

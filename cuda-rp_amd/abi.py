@@ -6,7 +6,7 @@ tests/test_abi.py checks sizes and offsets against the compiled library.
 """
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 ERR_INVALID = 1
@@ -75,7 +75,7 @@ class StepStats(C.Structure):
     _fields_ = [
         ("nstep", C.c_int64), ("time", C.c_double), ("resimax0", C.c_double),
         ("resimax", C.c_double), ("n_pic", C.c_int32), ("rp_max_substeps", C.c_int32),
-        ("rp_mean_substeps", C.c_double), ("status", C.c_int32), ("pad_", C.c_int32),
+        ("rp_mean_substeps", C.c_double), ("status", C.c_int32), ("rp_lean_launches", C.c_int32),
         ("focal_x", C.c_double), ("focal_y", C.c_double),
     ]
 

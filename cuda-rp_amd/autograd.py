@@ -25,21 +25,25 @@ class FocalObjective(torch.autograd.Function):
             r = sim.adjoint(box, n_steps, window, checkpoint_every, n_elem=amp.numel())
         finally:
             sim.close()
-        ga = torch.from_numpy(np.array(r["grad_amp"])).to(amp.dtype)
-        gp = torch.from_numpy(np.array(r["grad_phase"])).to(phase.dtype)
-        gf = torch.from_numpy(r["grad_fg0"]).to(fg0.dtype) if fg0 is not None else None
+        # gradients live where (and as what) their inputs do
+        ga = torch.from_numpy(np.array(r["grad_amp"])).to(device=amp.device, dtype=amp.dtype)
+        gp = torch.from_numpy(np.array(r["grad_phase"])).to(device=phase.device,
+                                                             dtype=phase.dtype)
+        gf = (torch.from_numpy(np.array(r["grad_fg0"])).to(device=fg0.device, dtype=fg0.dtype)
+              if fg0 is not None else None)
         ctx.save_for_backward(ga, gp, *([gf] if gf is not None else []))
         ctx.has_fg = gf is not None
         ctx.stats = {k: r[k] for k in ("forward_ms", "reverse_ms", "memory_bytes",
                                        "n_checkpoints", "segment")}
-        return torch.tensor(r["J"], dtype=torch.float64)
+        return torch.tensor(r["J"], dtype=torch.float64, device=amp.device)
 
     @staticmethod
     def backward(ctx, gJ):
         saved = ctx.saved_tensors
         ga, gp = saved[0], saved[1]
-        gf = saved[2] * gJ if ctx.has_fg else None
-        return gJ * ga, gJ * gp, gf, None, None, None, None, None
+        gf = saved[2] * gJ.to(saved[2].device) if ctx.has_fg else None
+        return (gJ.to(ga.device) * ga, gJ.to(gp.device) * gp, gf, None, None, None, None,
+                None)
 
 
 def focal_objective(amp, phase, fg0, make_sim, box, n_steps, window, checkpoint_every=0):

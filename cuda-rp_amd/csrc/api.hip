@@ -115,6 +115,8 @@ struct brp_handle {
     int N = 0, M = 0, pitch = 0, rows = 0;
     size_t frame = 0;  // elements per frame
     double T_inf = 0, P_inf = 0, rho_inf = 0, cs_inf = 0, dt = 0, dt0_rp = 0;
+    int device = 0;                   // HIP device of the handle (current at brp_create)
+    int cus = 1, per_cu = 1;          // compute units; RP kernel workgroups per CU
     int last_sub_max = 0;             // longest RP sub-step chain so far in the last step
     double last_sub_per_launch = 0.0;  // RP sub-steps per launch in the last step (grid choice)
     double fp_x = 0, fp_y = 0, omega = 0, ctrl_x = 0, ctrl_y = 0, ctrl_dist = 0;
@@ -713,6 +715,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
             if (h->cfg.tr_cylindrical && h->W.i0 <= 1 && h->W.i0 <= h->W.i1) h->W.i0 = 0;  // axis fold target
         }
         int status = BRP_OK;
+        long long lost = 0;  // bubbles the RP kernel did not write back
         int loop = 0;
         int target = h->fz, prev = old_m;
         double bub_ms = 0.0, pic_p_ms = 0.0, other_ms = 0.0;
@@ -733,11 +736,14 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
                 if (tin)  // replay: the void fraction from the taped radii, no RP
                     launch_scatter_radii(s, h->g, h->bp, h->b, tin->Rt_it[loop], h->fgb[target], h->sc_d);
-                else
+                else {
+                    const bool lean = bubble_step_lean(h->last_sub_max, h->last_sub_per_launch,
+                                                       h->bp.extreme_threshold, h->cus);
+                    st.rp_lean_launches += lean ? 1 : 0;
                     launch_bubble_step<F>(s, *h->rpl_h, h->rpl_d, h->rp_cur, h->pl_m, h->pl_n,
-                                          h->pl_p, p0, h->fgb[target], h->sc_d, loop,
-                                          bubble_step_lean(h->last_sub_max, h->last_sub_per_launch,
-                                                           h->bp.extreme_threshold));
+                                          h->pl_p, p0, h->fgb[target], h->sc_d, loop, lean,
+                                          h->cus, h->per_cu);
+                }
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
                 if (rec && rec->want_grid)
                     HIPCHK(h, hipMemcpyAsync(rec->fg_it[loop], h->fgb[target], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
@@ -764,6 +770,12 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 resimax = r / h->P_inf;
                 h->last_sub_max = h->sc_h->sub_max;
                 h->last_sub_per_launch = (double)h->sc_h->sub_sum / loop;
+                // every bubble is written back exactly once per RP launch
+                if (!tin && h->sc_h->done != (unsigned)h->nb * (unsigned)loop) {
+                    status = BRP_ERR_STATE;
+                    lost = (long long)h->nb * loop - (long long)h->sc_h->done;
+                    break;
+                }
                 std::swap(target, prev);
                 float a = 0.f, c = 0.f, o = 0.f;
                 (void)hipEventElapsedTime(&o, h->ev[7], h->ev[4]);
@@ -847,6 +859,9 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                   : 0.0;
         st.status = status;
         if (stats) stats[step] = st;
+        if (status == BRP_ERR_STATE)
+            return set_err(h, status, "step %lld: the RP launch wrote back %lld bubbles fewer than "
+                           "it was given", (long long)h->nstep, lost);
         if (status != BRP_OK)
             return set_err(h, status, "step %lld: %s (flags %d, Picard %d)", (long long)h->nstep,
                            status == BRP_ERR_PICARD_CAP ? "Picard cap reached"
@@ -1180,6 +1195,9 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     if (h->frame > (size_t)INT32_MAX)  // the RP scatter keys cells by 32-bit frame offset
         return fail(set_err(h, BRP_ERR_INVALID, "grid of %d x %d cells exceeds 2^31 frame elements",
                             h->N, h->M));
+    if (hipGetDevice(&h->device) != hipSuccess)
+        return fail(set_err(h, BRP_ERR_DEVICE, "hipGetDevice failed"));
+    rp_grid_query(h->device, &h->cus, &h->per_cu);
     if (hipStreamCreate(&h->stream) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipStreamCreate failed"));
     h->ev.resize(8);
@@ -1591,6 +1609,74 @@ int brp_get_bubbles(brp_handle *h, int32_t which, double *dst) {
     return BRP_OK;
 }
 
+int brp_set_bubble_array(brp_handle *h, int32_t which, const double *src) {
+    if (!h || !src || which < 0 || which >= BRP_BUB_COUNT) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    const int n = h->nb;
+    if (n == 0) return BRP_OK;
+    const int c = h->rp_cur;
+    double *dst = nullptr;
+    switch (which) {
+        case BRP_BUB_X: dst = h->b.x; break;
+        case BRP_BUB_Y: dst = h->b.y; break;
+        case BRP_BUB_R_T: dst = h->b.R_t; break;
+        case BRP_BUB_R_P: dst = h->b.R[c]; break;
+        case BRP_BUB_R_N: dst = h->b.Rn[c]; break;
+        case BRP_BUB_D1R: dst = h->b.d1R[c]; break;
+        case BRP_BUB_PG: dst = h->b.PG[c]; break;
+        case BRP_BUB_PL_P: dst = h->b.PL[h->pl_p]; break;
+        case BRP_BUB_PL_N: dst = h->b.PL[h->pl_n]; break;
+        case BRP_BUB_PL_M: dst = h->b.PL[h->pl_m]; break;
+        case BRP_BUB_Q_B: dst = h->b.Q_B; break;
+        case BRP_BUB_DT: dst = h->b.bdt[c]; break;
+        case BRP_BUB_RE: dst = h->b.re[c]; break;
+        case BRP_BUB_VX: dst = h->b.vx; break;
+        case BRP_BUB_VY: dst = h->b.vy; break;
+        default: return set_err(h, BRP_ERR_INVALID, "bubble array %d is not settable", which);
+    }
+    std::vector<double> v(n);
+    for (int k = 0; k < n; k++) {
+        v[k] = src[h->perm[k]];
+        if (!std::isfinite(v[k])) return set_err(h, BRP_ERR_INVALID, "bubble %d: non-finite value", h->perm[k]);
+    }
+    if (which == BRP_BUB_X || which == BRP_BUB_Y) {
+        // the moved stencils join the void-fraction window
+        std::vector<double> other(n);
+        HIPCHK(h, hipMemcpy(other.data(), which == BRP_BUB_X ? h->b.y : h->b.x, sizeof(double) * n,
+                            hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; k++) {
+            const double px = which == BRP_BUB_X ? v[k] : other[k];
+            const double py = which == BRP_BUB_X ? other[k] : v[k];
+            const int ix = host_bin(px, h->g.rdx, 0.5), iy = host_bin(py, h->g.rdy, 0.5);
+            window_union(h->W, std::max(ix + h->bp.mbs, 0), std::min(ix + h->bp.mbe, h->N + 1),
+                         std::max(iy + h->bp.mbs, 0), std::min(iy + h->bp.mbe, h->M + 1));
+        }
+        if (h->cfg.tr_cylindrical && h->W.i0 <= 1 && h->W.i0 <= h->W.i1) h->W.i0 = 0;
+    }
+    HIPCHK(h, hipMemcpy(dst, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return BRP_OK;
+}
+
+int brp_get_clock(const brp_handle *h, int64_t *nstep, double *time, double *carry) {
+    if (!h) return BRP_ERR_INVALID;
+    if (nstep) *nstep = h->nstep;
+    if (time) *time = h->t;
+    if (carry) *carry = h->tx;
+    return BRP_OK;
+}
+
+int brp_set_clock(brp_handle *h, int64_t nstep, double time, double carry) {
+    if (!h || nstep < 0 || !std::isfinite(time) || !std::isfinite(carry)) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    h->nstep = nstep;
+    h->t = time;
+    h->tx = carry;
+    h->first_step = nstep == 0 ? 1 : 0;
+    return BRP_OK;
+}
+
 int brp_save_state(brp_handle *h, int32_t slot) {
     if (!h || slot < 0 || slot >= 16) return BRP_ERR_INVALID;
     int rc = brp_prepare(h);
@@ -1617,11 +1703,25 @@ int brp_adjoint(brp_handle *h, const brp_objective *obj, double *grad_amp, doubl
     if (grad_fg0 && ld < h->N + 2) return set_err(h, BRP_ERR_INVALID, "adjoint: ld < N + 2");
     if (h->cfg.tr_cylindrical || h->cfg.sim_order != 2)
         return set_err(h, BRP_ERR_INVALID, "adjoint: Cartesian, simulation.order = 2 only");
+    // The reverse RP kernel linearises the shortened-chain sub-step; with
+    // rp_exact = 1 the forward integrates the operation-by-operation form,
+    // whose sub-step counts can differ, so the gradient would follow another
+    // trajectory than the one run.
+    if (h->cfg.rp_exact && h->cfg.bub_enable)
+        return set_err(h, BRP_ERR_INVALID, "adjoint: rp_exact = 1 is not differentiated "
+                       "(use rp_exact = 0)");
     int rc = brp_prepare(h);
     if (rc) return rc;
-    return h->prec == BRP_PREC_FP64
-               ? adjoint_impl<double>(h, o, grad_amp, grad_phase, grad_fg0, ld, stats)
-               : adjoint_impl<float>(h, o, grad_amp, grad_phase, grad_fg0, ld, stats);
+    brp_adjoint_stats local{};
+    brp_adjoint_stats *st = stats ? stats : &local;
+    rc = h->prec == BRP_PREC_FP64
+             ? adjoint_impl<double>(h, o, grad_amp, grad_phase, grad_fg0, ld, st)
+             : adjoint_impl<float>(h, o, grad_amp, grad_phase, grad_fg0, ld, st);
+    if (rc == BRP_OK && st->schedule_mismatch != 0)
+        return set_err(h, BRP_ERR_STATE, "adjoint: %lld replayed RP calls took another sub-step "
+                       "count than the forward (the gradient is not that of the run)",
+                       (long long)st->schedule_mismatch);
+    return rc;
 }
 
 int brp_set_focus(brp_handle *h, double fp_x, double fp_y, double f_dist) {

@@ -51,6 +51,14 @@ __host__ __device__ inline int64_t frame_off(int i, int j, int pitch) {
     return (int64_t)(j + 1) * pitch + (i + kCol0);
 }
 
+// Bit-exact binning: floor(x * rdx + c) with no FMA contraction
+// (BubbleUpdateIndexKernel, bubble_CUDA_kernel.cuh:355-356).
+__device__ inline int bin_index(double x, double rdx, double c) {
+#pragma clang fp contract(off)
+    double v = floor(x * rdx + c);
+    return (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
+}
+
 // Window of cells where any void-fraction buffer may be nonzero.
 struct Window {
     int i0, i1, j0, j1;  // inclusive, within [0, N+1]
@@ -63,6 +71,8 @@ struct StepScalars {
                                          // range / nonfinite R, bit 2 sub-step cap
     int sub_max;                         // max RP sub-steps this step
     unsigned long long sub_sum;          // sum of RP sub-steps this step
+    unsigned int done;                   // bubbles written back by the RP kernel this step
+    int pad_;
     double focal[2];                     // (x, y) of the largest temperature increment
     unsigned long long resid[kMaxEval];  // max |dp0| bit patterns (>= 0 doubles)
     unsigned int work[kMaxEval];         // bubble work-queue heads, one per Picard iteration
@@ -175,12 +185,15 @@ void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
 template <typename F>
 void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev, int cur,
                         int plm, int pln, int plp, const F *p0, int64_t *fg_target,
-                        StepScalars *sc, int slot, bool lean);
-int bubble_step_blocks(int n_bubbles, bool lean);
+                        StepScalars *sc, int slot, bool lean, int cus, int per_cu);
+// The device's compute units and the persistent RP kernel's workgroups per
+// compute unit (its occupancy limit), queried once per handle.
+void rp_grid_query(int device, int *cus, int *per_cu);
+int bubble_step_blocks(int n_bubbles, bool lean, int cus, int per_cu);
 // Whether the next RP launch is bound by its longest chains (lean grid): the
 // longest chain of the last launch is long and the sub-steps per lane of a
 // one-workgroup-per-CU grid are few against it.
-bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold);
+bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold, int cus);
 void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);

@@ -18,14 +18,6 @@ namespace {
 
 constexpr int kMaxBand = 8;
 
-// Bit-exact binning: floor(x * rdx + c) with no FMA contraction
-// (BubbleUpdateIndexKernel, bubble_CUDA_kernel.cuh:355-356).
-__device__ inline int bin_index(double x, double rdx, double c) {
-#pragma clang fp contract(off)
-    double v = floor(x * rdx + c);
-    return (v >= -1.0e9 && v <= 1.0e9) ? (int)v : -1000000000;
-}
-
 // smooth_delta_x / smooth_delta_y (bubble_CUDA_kernel.cuh:214-222)
 __device__ inline double delta_w(double coef_x, double coef_k, int pos, double h, double b) {
 #pragma clang fp contract(off)
@@ -400,6 +392,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     int ibnx = 0, ibny = 0;
     int sub_max = 0;
     unsigned long long sub_sum = 0;
+    unsigned int done = 0;  // bubbles this lane wrote back
     bool bad = false, capped_any = false;
     while (true) {
         if (solo && __ballot(idx >= 0) == 0ull) {
@@ -517,6 +510,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
                 b.R_t[idx] = R_t;
                 b.Q_B[idx] = Q_B;
                 b.sub[idx] = cnt;
+                done++;
                 sub_max = max(sub_max, cnt);
                 sub_sum += (unsigned long long)cnt;
                 capped_any |= capped && remain > 0.0;
@@ -532,8 +526,10 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     for (int off = 32; off > 0; off >>= 1) {
         sub_max = max(sub_max, __shfl_xor(sub_max, off, 64));
         sub_sum += __shfl_xor(sub_sum, off, 64);
+        done += __shfl_xor(done, off, 64);
     }
     if (lane == 0) {
+        if (done) atomicAdd(&sc->done, done);
         if (sub_sum) atomicAdd(&sc->sub_sum, sub_sum);
         if (sub_max > __hip_atomic_load(&sc->sub_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&sc->sub_max, sub_max);
@@ -629,46 +625,40 @@ void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
 // lean 23.0 vs full 26.2 ms of RP per step; 4096^2, 2^22 clustered bubbles
 // at 3 MPa (a 4000-sub-step chain, several thousand per lane): lean 84 vs
 // full 60 ms.  bubble_step_lean decides from the previous launch.
-static int g_cus = 0, g_per_cu = 0;
-static void rp_grid_init() {
-    if (!g_cus) {
-        int dev = 0, c = 0, p = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_bubble_persistent<float, true>,
-                                                           256, 0);
-        g_per_cu = std::max(1, p);
-        g_cus = std::max(1, c);
-    }
+void rp_grid_query(int device, int *cus, int *per_cu) {
+    int c = 0, p = 0;
+    (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, device);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_bubble_persistent<float, true>, 256,
+                                                       0);
+    *cus = std::max(1, c);
+    *per_cu = std::max(1, p);
 }
 
-bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold) {
-    rp_grid_init();
-    const double lanes = 256.0 * g_cus;  // lanes of the lean grid
+bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold, int cus) {
+    const double lanes = 256.0 * cus;  // lanes of the lean grid
     return sub_max >= 2 * extreme_threshold && sub_per_launch < 0.35 * lanes * sub_max;
 }
 
-int bubble_step_blocks(int n_bubbles, bool lean) {
-    rp_grid_init();
-    return std::max(1, std::min(g_cus * (lean ? 1 : g_per_cu), (n_bubbles + 255) / 256));
+int bubble_step_blocks(int n_bubbles, bool lean, int cus, int per_cu) {
+    return std::max(1, std::min(cus * (lean ? 1 : per_cu), (n_bubbles + 255) / 256));
 }
 
 template <typename F>
 void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev, int cur,
                         int plm, int pln, int plp, const F *p0, int64_t *fg_target,
-                        StepScalars *sc, int slot, bool lean) {
+                        StepScalars *sc, int slot, bool lean, int cus, int per_cu) {
     if (host.b.n <= 0) return;
-    const dim3 grid(bubble_step_blocks(host.b.n, lean));
+    const dim3 grid(bubble_step_blocks(host.b.n, lean, cus, per_cu));
     const int cap = host.bp.max_substeps;
     // Dedicated wavefronts: up to one per workgroup on the lean grid, where
-    // the longest chains bound the launch; g_cus / 8 on the full grid, which
+    // the longest chains bound the launch; CUs / 8 on the full grid, which
     // is chosen when the total work bounds it: a wavefront with one active
     // lane still takes its SIMD's issue slots, so every dedicated wavefront
     // costs a SIMD of throughput there.  4096^2, 2^22 clustered bubbles
     // (full grid): 512 dedicated 49.8, 128 46.8, 32 46.4, 0 46.2 ms/step;
     // with refills batched by kRefillBatch: 128 29.6, 64 29.2, 32 29.3,
     // 0 30.8; 2048^2 (lean grid): 256 17.2, 32 17.4, 0 19.1 ms/step.
-    const int solo_cap = lean ? (int)grid.x : std::max(1, g_cus / 8);
+    const int solo_cap = lean ? (int)grid.x : std::max(1, cus / 8);
     if (host.bp.rp_exact)
         hipLaunchKernelGGL((k_bubble_persistent<F, false>), grid, dim3(256), 0, s, dev, cap,
                            host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot,
@@ -901,9 +891,9 @@ template void launch_motion<float>(hipStream_t, const GridParams &, const Bubble
                                    StepScalars *);
 template void launch_bubble_step<double>(hipStream_t, const RPLaunch &, const RPLaunch *, int, int,
                                          int, int, const double *, int64_t *, StepScalars *,
-                                         int, bool);
+                                         int, bool, int, int);
 template void launch_bubble_step<float>(hipStream_t, const RPLaunch &, const RPLaunch *, int, int,
                                          int, int, const float *, int64_t *, StepScalars *,
-                                         int, bool);
+                                         int, bool, int, int);
 
 }  // namespace brp

@@ -355,6 +355,7 @@ __global__ void k_zero_scalars(StepScalars *sc, int nslots) {
         sc->flags = 0;
         sc->sub_max = 0;
         sc->sub_sum = 0ull;
+        sc->done = 0u;
     }
 }
 

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define BRP_ABI_VERSION 1
+#define BRP_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -123,7 +123,8 @@ typedef struct brp_step_stats {
     int32_t rp_max_substeps;     /* max RP sub-steps of any bubble this step */
     double rp_mean_substeps;     /* mean RP sub-steps per bubble per call    */
     int32_t status;              /* BRP_OK or the error of this step         */
-    int32_t pad_;
+    int32_t rp_lean_launches;    /* RP launches of this step on the lean grid
+                                    (one workgroup per compute unit)         */
     /* With thermal = 1: the cell (x = i dx, y = j dy) of the largest
      * temperature increment of the step over the 2m grid, the raw input of
      * the reference's focal-point tracking (determine_focal_point on Work
@@ -243,6 +244,23 @@ int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld);
 /* Copies one bubble array (n_bubbles values, in the order given to
  * brp_set_bubbles) into dst. */
 int brp_get_bubbles(brp_handle *h, int32_t which, double *dst);
+
+/* Overwrites one bubble array (n_bubbles host doubles in the order given to
+ * brp_set_bubbles): X, Y, R_T, R_P, R_N, D1R, PG, PL_P, PL_N, PL_M, Q_B, DT,
+ * RE, VX or VY -- the state a step starts from (bubble_t,
+ * bubble_CUDA.h:172-189), where R_P/R_N/D1R/PG/DT/RE are the state after the
+ * last step, before store_variables shifts it (kernel.cuh:1229-1262).  With
+ * brp_set_field and brp_set_clock it moves a run to a state taken from
+ * another run (a restart, or the oracle seeded with a state the GPU reached).
+ * No reference counterpart (the reference restarts from nothing). */
+int brp_set_bubble_array(brp_handle *h, int32_t which, const double *src);
+
+/* The compensated clock (bubble_CUDA.cu:68-69, 208-212): step counter, time
+ * and its two-sum carry.  brp_set_clock with nstep > 0 also marks the first
+ * step as done (the mixture density is then (1 - f_g) rho, not rho).  No
+ * reference counterpart (see brp_set_bubble_array). */
+int brp_get_clock(const brp_handle *h, int64_t *nstep, double *time, double *carry);
+int brp_set_clock(brp_handle *h, int64_t nstep, double time, double carry);
 
 /* Full-state snapshot into slot (0..15) kept in device memory, and its
  * restore.  The snapshot holds every field, every bubble array, the clock

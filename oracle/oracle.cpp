@@ -1502,6 +1502,53 @@ int oracle_get_bubbles(const oracle *o, int32_t which, double *dst) {
     return BRP_OK;
 }
 
+// State injection (the counterpart of brp_set_bubble_array): one bubble
+// array of the state a step starts from; positions re-bin the bubble.
+int oracle_set_bubble_array(oracle *o, int32_t which, const double *src) {
+    if (!o || !src) return BRP_ERR_INVALID;
+    if (!o->prepared) oracle_prepare(o);
+    std::vector<double> *v = nullptr;
+    switch (which) {
+        case BRP_BUB_X: v = &o->bx; break;
+        case BRP_BUB_Y: v = &o->by; break;
+        case BRP_BUB_R_T: v = &o->R_t; break;
+        case BRP_BUB_R_P: v = &o->R_p; break;
+        case BRP_BUB_R_N: v = &o->R_n; break;
+        case BRP_BUB_D1R: v = &o->d1_R_p; break;
+        case BRP_BUB_PG: v = &o->PG_p; break;
+        case BRP_BUB_PL_P: v = &o->PL_p; break;
+        case BRP_BUB_PL_N: v = &o->PL_n; break;
+        case BRP_BUB_PL_M: v = &o->PL_m; break;
+        case BRP_BUB_Q_B: v = &o->Q_B; break;
+        case BRP_BUB_DT: v = &o->bdt; break;
+        case BRP_BUB_RE: v = &o->re; break;
+        case BRP_BUB_VX: v = &o->vBx; break;
+        case BRP_BUB_VY: v = &o->vBy; break;
+        default: return BRP_ERR_INVALID;
+    }
+    std::memcpy(v->data(), src, sizeof(double) * o->nb);
+    if (which == BRP_BUB_X || which == BRP_BUB_Y)
+        for (int b = 0; b < o->nb; b++) bin_bubble(o, b);
+    return BRP_OK;
+}
+
+int oracle_get_clock(const oracle *o, int64_t *nstep, double *time, double *carry) {
+    if (!o) return BRP_ERR_INVALID;
+    if (nstep) *nstep = o->nstep;
+    if (time) *time = o->tstep;
+    if (carry) *carry = o->tstepx;
+    return BRP_OK;
+}
+
+int oracle_set_clock(oracle *o, int64_t nstep, double time, double carry) {
+    if (!o || nstep < 0) return BRP_ERR_INVALID;
+    if (!o->prepared) oracle_prepare(o);
+    o->nstep = nstep;
+    o->tstep = time;
+    o->tstepx = carry;
+    return BRP_OK;
+}
+
 int oracle_array_index(const oracle *o, int32_t out[28]) {
     if (!o) return BRP_ERR_INVALID;
     const ArrayIndex &a = o->a;

@@ -48,6 +48,10 @@ const char *oracle_last_error(const oracle *o);
  * replay position; mode 1 clears the record.  schedule_copy hands a record to
  * another instance with the same bubble count. */
 int oracle_schedule_mode(oracle *o, int32_t mode);
+/* brp_set_bubble_array, brp_get_clock, brp_set_clock of the oracle. */
+int oracle_set_bubble_array(oracle *o, int32_t which, const double *src);
+int oracle_get_clock(const oracle *o, int64_t *nstep, double *time, double *carry);
+int oracle_set_clock(oracle *o, int64_t nstep, double time, double carry);
 /* brp_set_focus of the oracle. */
 int oracle_set_focus(oracle *o, double fp_x, double fp_y, double f_dist);
 int oracle_schedule_copy(oracle *dst, const oracle *src);

@@ -114,3 +114,87 @@ def focal_objective(sim, steps, box, window):
             p = sim.field("p0")[j0:j1 + 1, i0:i1 + 1]
             acc += float(np.sum(p * p))
     return acc / (window * (i1 - i0 + 1) * (j1 - j0 + 1))
+
+
+# ---- the benchmark configurations (BASELINE configs C1-C5) -----------------
+FOCAL = 15e-3
+
+
+def _focus(cfg):
+    return cfg.ext_focus_x, FOCAL * 0.5 * np.sqrt(3.0)
+
+
+def c1_case(precision=abi.PREC_FP64, seed=11):
+    """256^2 cells (dx = dy = 0.1 mm), 16-cell PML on all sides, water, a
+    64-element focused aperture on the y0 edge (focal length 15 mm) driven by a
+    1 MHz CW sinusoid of 0.5 MPa, 1024 bubbles with R0 ~ U(1, 5) um uniform in
+    a 4 x 4 mm box at the focus, mean void fraction 1e-6 over the box, CFL 0.5
+    (2000 steps in the full case)."""
+    cfg = abi.water_config(256, npml=16, amp_pa=0.5e6, freq=1.0e6, focal_dist=FOCAL,
+                           fg0=1e-6, precision=precision)
+    r = rng(seed)
+    fx, fy = _focus(cfg)
+    nb, box = 1024, 4e-3
+    x = fx + (r.random(nb) - 0.5) * box
+    y = fy + (r.random(nb) - 0.5) * box
+    R0 = 1e-6 + 4e-6 * r.random(nb)
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    source = (np.full(64, 0.5e6), np.zeros(64))
+    return cfg, (x, y, R0, None, fg_in), source
+
+
+def c2_case(precision=abi.PREC_FP64, n=2048, nb=1 << 20, seed=2):
+    """2048^2, 32-cell PML, 1 MHz 1.5 MPa focused CW, 2^20 bubbles with
+    log-normal R0 (median 3 um, sigma_log 0.4) uniform in an 8 x 8 mm box,
+    mean void fraction 1e-6 over the box."""
+    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=1.5e6, freq=1.0e6, focal_dist=FOCAL,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = rng(seed)
+    fx, fy = _focus(cfg)
+    box = 8e-3
+    x = fx + (r.random(nb) - 0.5) * box
+    y = fy + (r.random(nb) - 0.5) * box
+    R0 = np.exp(np.log(3e-6) + 0.4 * r.standard_normal(nb))
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    return cfg, (x, y, R0, None, fg_in)
+
+
+def c3_case(precision=abi.PREC_MIXED, n=4096, nb=1 << 22, seed=3):
+    """4096^2, 32-cell PML, 2 MHz 3 MPa focused CW, 2^22 log-normal bubbles
+    (median 3 um, sigma_log 0.4) clustered in 16 Gaussian clouds (centres
+    uniform over an 8 x 8 mm box at the focus, sigma 0.25 mm, clipped to the
+    box), mean void fraction 1e-6 over the box."""
+    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=3.0e6, freq=2.0e6, focal_dist=FOCAL,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = rng(seed)
+    fx, fy = _focus(cfg)
+    box = 8e-3
+    cx = fx + (r.random(16) - 0.5) * box
+    cy = fy + (r.random(16) - 0.5) * box
+    k = r.integers(0, 16, nb)
+    x = np.clip(cx[k] + 0.25e-3 * r.standard_normal(nb), fx - 0.5 * box, fx + 0.5 * box)
+    y = np.clip(cy[k] + 0.25e-3 * r.standard_normal(nb), fy - 0.5 * box, fy + 0.5 * box)
+    R0 = np.exp(np.log(3e-6) + 0.4 * r.standard_normal(nb))
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    return cfg, (x, y, R0, None, fg_in)
+
+
+def c4_case(precision=abi.PREC_MIXED, n=1024, nb=65536, n_elem=64, seed=4):
+    """1024^2, 32-cell PML, 65536 bubbles with R0 ~ U(1, 5) um in a 4 x 4 mm
+    box at the focus (mean void fraction 1e-6 there), a 64-element aperture
+    with amplitudes ~ U(0.3, 0.7) MPa and phases ~ U(0, 2 pi), 1 MHz; the
+    objective box is 2 x 2 mm (20 x 20 cells) at the focus."""
+    cfg = abi.water_config(n, npml=32, cfl=0.5, amp_pa=0.5e6, freq=1.0e6, focal_dist=FOCAL,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = rng(seed)
+    fx, fy = _focus(cfg)
+    box = 4e-3
+    x = fx + (r.random(nb) - 0.5) * box
+    y = fy + (r.random(nb) - 0.5) * box
+    R0 = 1e-6 + 4e-6 * r.random(nb)
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    amp = 0.3e6 + 0.4e6 * r.random(n_elem)
+    phase = 2.0 * np.pi * r.random(n_elem)
+    dx = cfg.grid_lx / n
+    ic, jc = int(round(fx / dx + 0.5)), int(round(fy / dx + 0.5))
+    return cfg, (x, y, R0, None, fg_in), (amp, phase), (ic - 10, ic + 9, jc - 10, jc + 9)

@@ -64,6 +64,9 @@ def load():
         "oracle_schedule_mode": (C.c_int, [vp, i32]),
         "oracle_schedule_copy": (C.c_int, [vp, vp]),
         "oracle_set_focus": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "oracle_set_bubble_array": (C.c_int, [vp, i32, dp]),
+        "oracle_get_clock": (C.c_int, [vp, C.POINTER(i64), dp, dp]),
+        "oracle_set_clock": (C.c_int, [vp, i64, C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -235,9 +235,14 @@ def test_many_wavefront_schedule_bit_exact(brp):
         runs.append((st, g.field("p0"), g.field("fg"), g.bubbles("R_t"), g.bubbles("sub")))
         g.close()
     (s1, *a1), (s2, *a2) = runs
-    print("max sub-steps", max(s["rp_max_substeps"] for s in s1), flush=True)
+    lean = sum(s["rp_lean_launches"] for s in s1)
+    launches = sum(s["n_pic"] for s in s1)
+    print("max sub-steps", max(s["rp_max_substeps"] for s in s1), "lean launches", lean, "of",
+          launches, flush=True)
     assert max(s["rp_max_substeps"] for s in s1) > 512  # long chains, dedicated wavefronts
+    assert 0 < lean < launches  # both grid shapes were used
     assert [s["n_pic"] for s in s1] == [s["n_pic"] for s in s2]
+    assert [s["rp_lean_launches"] for s in s1] == [s["rp_lean_launches"] for s in s2]
     for x, y in zip(a1, a2):
         np.testing.assert_array_equal(x, y)
 
@@ -272,6 +277,30 @@ def test_error_paths(brp):
     bad.sim_order = 4
     with pytest.raises(brp.BrpError):
         brp.Simulation(bad)
+
+
+@pytest.mark.parametrize("where", ["p0", "R"])
+def test_nonfinite_is_reported(brp, where):
+    """BRP_ERR_NONFINITE: a NaN planted in the pressure at the bubble box, or
+    a bubble radius set to zero (its RP sub-step then divides by zero), is
+    reported by the step that meets it; every kernel still ends normally
+    (the sub-step sizes stay finite, so each RP integration ends)."""
+    cfg, b = cases.bubbly_case(48, nb=50)
+    g = cases.make(brp.Simulation, cfg, b)
+    g.step(5)
+    if where == "p0":
+        p = g.field("p0")
+        ny, nx = p.shape
+        p[ny // 2, nx // 2] = np.nan
+        g.set_field("p0", p)
+    else:
+        R = g.bubbles("R_p")
+        R[7] = 0.0
+        g.set_bubble_array("R_p", R)
+    with pytest.raises(brp.BrpError) as e:
+        g.step(20)
+    assert e.value.code == brp.abi.ERR_NONFINITE, str(e.value)
+    g.close()
 
 
 def test_standing_wave_closed_form(brp):
