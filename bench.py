@@ -430,6 +430,7 @@ def main():
         "picard_per_step": sm["picard_per_step"],
         "rp_max_substeps": sm["rp_max_substeps"],
         "rp_lean_launches": sm["rp_lean_launches"],
+        "timed_picard_counts": [s["n_pic"] for s in st_m],
         "phase_ms_per_step": sm["phase_ms_per_step"],
         "roofline": rp_roofline(sm, nb, measured_traffic("k_bubble_persistent_4096_counters.md")),
         "step_roofline": step_roofline(sm, "mixed"),

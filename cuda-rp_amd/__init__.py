@@ -59,6 +59,9 @@ def load_library():
         "brp_get_phase_times": (C.c_int, [vp, dp]),
         "brp_set_focus": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "brp_set_bubble_array": (C.c_int, [vp, i32, dp]),
+        "brp_scatter_variant": (C.c_int, [vp, i32, dp, i64]),
+        "brp_schedule_mode": (C.c_int, [vp, i32]),
+        "brp_schedule_copy": (C.c_int, [vp, vp]),
         "brp_get_clock": (C.c_int, [vp, C.POINTER(i64), dp, dp]),
         "brp_set_clock": (C.c_int, [vp, i64, C.c_double, C.c_double]),
         "brp_adjoint": (C.c_int, [vp, C.POINTER(abi.Objective), dp, dp, dp, i64,
@@ -241,6 +244,22 @@ class Simulation(_Backend):
                 "forward_ms": st.forward_ms, "reverse_ms": st.reverse_ms,
                 "memory_bytes": st.memory_bytes, "n_checkpoints": st.n_checkpoints,
                 "segment": st.segment, "schedule_mismatch": st.schedule_mismatch}
+
+    def schedule_mode(self, mode):
+        """0 reference behaviour, 1 record, 2 replay the recorded schedule."""
+        self._check(self.lib.brp_schedule_mode(self.h, mode))
+
+    def schedule_from(self, other):
+        """Replay other's recorded schedule (shared, same device)."""
+        self._check(self.lib.brp_schedule_copy(self.h, other.h))
+
+    def scatter_variant(self, mode):
+        """brp_scatter_variant: the current R_t scattered with accumulation
+        scheme `mode` (abi.VF_*), as an f_g-shaped array."""
+        nx, ny = self.shape("fg")[:2]
+        out = np.zeros((ny, nx))
+        self._check(self.lib.brp_scatter_variant(self.h, mode, _dptr(out), nx))
+        return out
 
     def phase_times(self):
         a = (C.c_double * 5)()

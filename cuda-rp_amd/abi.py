@@ -20,6 +20,8 @@ ERR_STATE = 7
 PREC_FP64 = 0
 PREC_MIXED = 1
 
+VF_INT64, VF_FP64_ATOMIC, VF_FP32_COMPENSATED, VF_FP32_ATOMIC = range(4)
+
 FIELD_P0, FIELD_PX, FIELD_PY, FIELD_VX, FIELD_VY, FIELD_FG, FIELD_FGN, FIELD_FGM, \
     FIELD_RHO_M, FIELD_T = range(10)
 FIELD_NAMES = {"p0": FIELD_P0, "px": FIELD_PX, "py": FIELD_PY, "vx": FIELD_VX,

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -92,6 +93,32 @@ struct StepRecord {
     double t = 0.0;
 };
 
+// A recorded frozen schedule: each step's Picard count and, for every RP
+// launch in order, each bubble's accepted step sizes (dts[off[i] .. off[i+1])
+// for bubble i in the library's cell-sorted order).  Immutable once recorded;
+// handles replaying it share it.
+struct Schedule {
+    struct Call {
+        int64_t *off = nullptr;
+        double *dts = nullptr;
+    };
+    int nb = 0;
+    int device = 0;
+    std::vector<int> npic;
+    std::vector<Call> calls;
+    std::vector<void *> bufs;
+    unsigned int *mismatch = nullptr;  // device counter of recordings that took another count
+    ~Schedule() {
+        for (void *q : bufs) (void)hipFree(q);
+    }
+    void *alloc(size_t bytes) {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+        bufs.push_back(q);
+        return q;
+    }
+};
+
 // Optional control of run_steps: a forced Picard count per step (the frozen
 // schedule of a replay), a record of one step's intermediates, the Picard
 // counts taken, and a bubble tape to replay instead of integrating RP (the
@@ -159,6 +186,11 @@ struct brp_handle {
     std::vector<size_t> alloc_sizes;
     size_t dev_bytes = 0;
     Snapshot snaps[16];
+
+    // frozen schedule (brp_schedule_mode): 1 record, 2 replay
+    int sched_mode = 0;
+    std::shared_ptr<Schedule> sched;
+    size_t sched_step = 0, sched_call = 0;
 };
 
 namespace {
@@ -631,6 +663,32 @@ void rec_free(StepRecord &r) {
 // ---- the coupled step ----------------------------------------------------
 int rec_grow(brp_handle *h, StepRecord *r, int k);
 
+// Frozen-schedule recording of the RP launch just queued: its per-bubble
+// sub-step counts give the offsets, then k_rp_record recomputes the same
+// integrations and stores every accepted step size.
+int schedule_record_call(brp_handle *h) {
+    Schedule &S = *h->sched;
+    const int n = h->nb;
+    std::vector<int> sub(n);
+    HIPCHK(h, hipMemcpyAsync(sub.data(), h->b.sub, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<int64_t> off(n + 1, 0);
+    for (int k = 0; k < n; k++) off[k + 1] = off[k] + sub[k];
+    Schedule::Call cl;
+    cl.off = (int64_t *)S.alloc(sizeof(int64_t) * (n + 1));
+    cl.dts = (double *)S.alloc(sizeof(double) * std::max<int64_t>(off[n], 1));
+    if (!cl.off || !cl.dts)
+        return set_err(h, BRP_ERR_DEVICE, "frozen schedule: allocation of %lld step sizes failed",
+                       (long long)off[n]);
+    HIPCHK(h, hipMemcpyAsync(cl.off, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
+                             h->stream));
+    launch_rp_record(h->stream, h->rpl_d, n, h->rp_cur, h->pl_m, h->pl_n, h->pl_p, cl.off, cl.dts,
+                     S.mismatch);
+    HIPCHK(h, hipGetLastError());
+    S.calls.push_back(cl);
+    return BRP_OK;
+}
+
 template <typename F>
 int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullptr) {
     const double thr = (double)1.0e-7f;  // bubble_CUDA.cu:243
@@ -666,7 +724,13 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         }
         launch_zero_scalars(s, h->sc_d, max_pic + 1);
         StepRecord *rec = ctl ? ctl->rec : nullptr;
-        const int forced = (ctl && ctl->force_npic) ? ctl->force_npic[step] : -1;
+        int forced = (ctl && ctl->force_npic) ? ctl->force_npic[step] : -1;
+        if (h->sched_mode == 2) {
+            if (h->sched_step >= h->sched->npic.size())
+                return set_err(h, BRP_ERR_STATE, "step %lld: the frozen schedule is exhausted",
+                               (long long)h->nstep);
+            forced = h->sched->npic[h->sched_step++];
+        }
         if (rec) {
             int rc = rec_grow(h, rec, 1);
             if (rc) return rc;
@@ -732,17 +796,30 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 }
                 HIPCHK(h, hipEventRecord(h->ev[7], s));
                 const StepRecord *tin = ctl ? ctl->tape_in : nullptr;
-                if (!tin) launch_order_partition(s, h->b, h->bp.heavy_threshold, h->bp.extreme_threshold);
+                const bool replay = h->sched_mode == 2;
+                if (!tin && !replay)
+                    launch_order_partition(s, h->b, h->bp.heavy_threshold, h->bp.extreme_threshold);
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
-                if (tin)  // replay: the void fraction from the taped radii, no RP
+                if (tin) {  // replay: the void fraction from the taped radii, no RP
                     launch_scatter_radii(s, h->g, h->bp, h->b, tin->Rt_it[loop], h->fgb[target], h->sc_d);
-                else {
+                } else if (replay) {  // the recorded step sizes of this RP call
+                    if (h->sched_call >= h->sched->calls.size())
+                        return set_err(h, BRP_ERR_STATE, "step %lld: the frozen schedule has no "
+                                       "RP call %zu", (long long)h->nstep, h->sched_call);
+                    const Schedule::Call &cl = h->sched->calls[h->sched_call++];
+                    launch_rp_replay<F>(s, h->rpl_d, h->nb, h->rp_cur, h->pl_m, h->pl_n, h->pl_p,
+                                        p0, h->fgb[target], h->sc_d, cl.off, cl.dts);
+                } else {
                     const bool lean = bubble_step_lean(h->last_sub_max, h->last_sub_per_launch,
                                                        h->bp.extreme_threshold, h->cus);
                     st.rp_lean_launches += lean ? 1 : 0;
                     launch_bubble_step<F>(s, *h->rpl_h, h->rpl_d, h->rp_cur, h->pl_m, h->pl_n,
                                           h->pl_p, p0, h->fgb[target], h->sc_d, loop, lean,
                                           h->cus, h->per_cu);
+                    if (h->sched_mode == 1) {
+                        int rc = schedule_record_call(h);
+                        if (rc) return rc;
+                    }
                 }
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
                 if (rec && rec->want_grid)
@@ -853,15 +930,24 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
             }
         }
         if (ctl && ctl->npic_out) ctl->npic_out->push_back(loop);
+        if (h->sched_mode == 1) {
+            h->sched->npic.push_back(loop);
+            unsigned int mm = 0;
+            HIPCHK(h, hipMemcpy(&mm, h->sched->mismatch, sizeof mm, hipMemcpyDeviceToHost));
+            if (mm) status = BRP_ERR_STATE;
+        }
         st.rp_max_substeps = h->sc_h->sub_max;
         st.rp_mean_substeps = (loop > 0 && h->nb > 0)
                                   ? (double)h->sc_h->sub_sum / ((double)h->nb * loop)
                                   : 0.0;
         st.status = status;
         if (stats) stats[step] = st;
-        if (status == BRP_ERR_STATE)
+        if (status == BRP_ERR_STATE && lost)
             return set_err(h, status, "step %lld: the RP launch wrote back %lld bubbles fewer than "
                            "it was given", (long long)h->nstep, lost);
+        if (status == BRP_ERR_STATE)
+            return set_err(h, status, "step %lld: recording the schedule took another sub-step "
+                           "count than the RP launch", (long long)h->nstep);
         if (status != BRP_OK)
             return set_err(h, status, "step %lld: %s (flags %d, Picard %d)", (long long)h->nstep,
                            status == BRP_ERR_PICARD_CAP ? "Picard cap reached"
@@ -1658,6 +1744,83 @@ int brp_set_bubble_array(brp_handle *h, int32_t which, const double *src) {
     return BRP_OK;
 }
 
+int brp_scatter_variant(brp_handle *h, int32_t mode, double *dst, int64_t ld) {
+    if (!h || !dst || mode < BRP_VF_INT64 || mode > BRP_VF_FP32_ATOMIC) return BRP_ERR_INVALID;
+    int rc = brp_prepare(h);
+    if (rc) return rc;
+    int nx, ny, i0, j0;
+    field_extent(h, BRP_FIELD_FG, &nx, &ny, &i0, &j0);
+    if (ld < nx) return set_err(h, BRP_ERR_INVALID, "ld %lld < %d", (long long)ld, nx);
+    // The spare void-fraction buffer (zero between steps) holds the result:
+    // one int64 / double per cell, or two floats (sum, compensation).
+    int64_t *spare = h->fgb[h->fz];
+    hipStream_t s = h->stream;
+    if (mode == BRP_VF_INT64)
+        launch_scatter_radii(s, h->g, h->bp, h->b, h->b.R_t, spare, h->sc_d);
+    else
+        launch_scatter_variant(s, h->g, h->bp, h->b, mode, spare, (int64_t)h->frame);
+    if (h->cfg.tr_cylindrical && mode == BRP_VF_INT64) launch_cyl_fold(s, h->g, spare);
+    HIPCHK(h, hipGetLastError());
+    std::vector<double> fr(h->frame);
+    if (mode == BRP_VF_INT64) {
+        rc = read_frame_fg(h, spare, fr);
+    } else if (mode == BRP_VF_FP64_ATOMIC) {
+        rc = read_frame<double>(h, spare, fr);
+    } else {
+        std::vector<float> t(2 * h->frame);
+        HIPCHK(h, hipMemcpyAsync(t.data(), spare, sizeof(float) * 2 * h->frame,
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(h, hipStreamSynchronize(s));
+        for (size_t k = 0; k < h->frame; k++) fr[k] = (double)t[k] + (double)t[h->frame + k];
+    }
+    HIPCHK(h, hipMemsetAsync(spare, 0, sizeof(int64_t) * h->frame, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    if (rc) return rc;
+    if (h->cfg.tr_cylindrical && mode != BRP_VF_INT64)
+        for (int j = 0; j <= h->M + 1; j++) {  // the axis fold of the other modes, on the host
+            const int64_t a = frame_off(0, j, h->pitch), b = frame_off(1, j, h->pitch);
+            fr[a] = fr[b] = fr[a] + fr[b];
+        }
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) dst[(size_t)j * ld + i] = fr[frame_off(i + i0, j + j0, h->pitch)];
+    return BRP_OK;
+}
+
+int brp_schedule_mode(brp_handle *h, int32_t mode) {
+    if (!h || mode < 0 || mode > 2) return BRP_ERR_INVALID;
+    if (mode != 0 && h->cfg.rp_exact && h->cfg.bub_enable)
+        return set_err(h, BRP_ERR_INVALID, "schedule: rp_exact = 1 is not supported");
+    if (mode == 1) {
+        auto S = std::make_shared<Schedule>();
+        S->nb = h->nb;
+        S->device = h->device;
+        S->mismatch = (unsigned int *)S->alloc(sizeof(unsigned int));
+        if (!S->mismatch || hipMemset(S->mismatch, 0, sizeof(unsigned int)) != hipSuccess)
+            return set_err(h, BRP_ERR_DEVICE, "schedule: allocation failed");
+        h->sched = S;
+    } else if (mode == 2 && !h->sched) {
+        return set_err(h, BRP_ERR_STATE, "schedule: nothing recorded to replay");
+    }
+    if (mode == 2 && h->sched->nb != h->nb)
+        return set_err(h, BRP_ERR_STATE, "schedule: recorded with %d bubbles, handle has %d",
+                       h->sched->nb, h->nb);
+    h->sched_mode = mode;
+    h->sched_step = 0;
+    h->sched_call = 0;
+    return BRP_OK;
+}
+
+int brp_schedule_copy(brp_handle *dst, const brp_handle *src) {
+    if (!dst || !src || !src->sched) return BRP_ERR_INVALID;
+    if (dst->device != src->device || (dst->prepared ? dst->nb : (int)dst->pending_x.size()) !=
+                                          src->sched->nb)
+        return set_err(dst, BRP_ERR_INVALID, "schedule_copy: needs the same device and bubble count");
+    dst->sched = src->sched;
+    dst->sched_step = 0;
+    dst->sched_call = 0;
+    return BRP_OK;
+}
+
 int brp_get_clock(const brp_handle *h, int64_t *nstep, double *time, double *carry) {
     if (!h) return BRP_ERR_INVALID;
     if (nstep) *nstep = h->nstep;
@@ -1710,6 +1873,8 @@ int brp_adjoint(brp_handle *h, const brp_objective *obj, double *grad_amp, doubl
     if (h->cfg.rp_exact && h->cfg.bub_enable)
         return set_err(h, BRP_ERR_INVALID, "adjoint: rp_exact = 1 is not differentiated "
                        "(use rp_exact = 0)");
+    if (h->sched_mode != 0)
+        return set_err(h, BRP_ERR_STATE, "adjoint: set schedule mode 0 first");
     int rc = brp_prepare(h);
     if (rc) return rc;
     brp_adjoint_stats local{};

@@ -198,6 +198,18 @@ void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold,
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);
+// Frozen schedule: record the accepted RP step sizes of the last RP launch
+// (same inputs, same sub-step function) and replay them in a later run.
+void launch_rp_record(hipStream_t s, const RPLaunch *dev, int n, int cur, int plm, int pln,
+                      int plp, const int64_t *off, double *dts, unsigned int *mismatch);
+template <typename F>
+void launch_rp_replay(hipStream_t s, const RPLaunch *dev, int n, int cur, int plm, int pln,
+                      int plp, const F *p0, int64_t *fg_target, StepScalars *sc,
+                      const int64_t *off, const double *dts);
+// Scatter of the current R_t with accumulation mode 1 (FP64 atomics),
+// 2 (compensated FP32 atomics) or 3 (FP32 atomics) into out (zeroed).
+void launch_scatter_variant(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                            const BubbleArrays &b, int mode, void *out, int64_t frame);
 void launch_scatter_radii(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                           const BubbleArrays &b, const double *Rt, int64_t *fg, StepScalars *sc);
 void launch_copy_d(hipStream_t s, double *dst, const double *src, int n);

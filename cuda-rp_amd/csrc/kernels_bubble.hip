@@ -538,6 +538,116 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     if (capped_any) atomicOr(&sc->flags, 4);
 }
 
+// ---- frozen schedule (finite differences of the frozen-schedule map) ------
+// Record: every bubble's RP integration of this Picard iteration again, from
+// the same inputs (the RP start state, the pressure history and the PL_p the
+// RP launch gathered), with the same sub-step function, writing each accepted
+// step size to dts[off[idx] + k]; the count must equal the launch's (a
+// differing count is counted in *mismatch and its extra sizes are dropped).
+__global__ __launch_bounds__(256) void k_rp_record(const RPLaunch *kp, int cur, int plm, int pln,
+                                                    int plp, const int64_t *off, double *dts,
+                                                    unsigned int *mismatch) {
+    const RPLaunch &P = *kp;
+    __shared__ double vtab[kRpVSize];
+    rp_vtab_stage(P.bp.rp_shared, vtab);
+    __syncthreads();
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const BubbleArrays &b = P.b;
+    if (idx >= b.n) return;
+    const RPFastShared fs = rp_shared_load(P.bp.rp_shared, vtab);
+    const double R0 = b.R0[idx], PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
+    const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
+                         P.bp.T0 / (PG0 * R03)};
+    RPLaneF<double> L;
+    rp_begin_g(L, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
+               b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], b.PL[plp][idx],
+               P.g.dt, P.bp.PL0);
+    const int64_t o = off[idx], cnt = off[idx + 1] - o;
+    while (L.remain > 0.0 && L.counter < P.bp.max_substeps) {
+        rp_substep_g(L, fk, fs);
+        if (L.counter <= cnt) dts[o + L.counter - 1] = L.dtL;
+    }
+    if (L.counter != cnt) atomicAdd(mismatch, 1u);
+}
+
+// Replay: one Picard iteration of every bubble with the recorded step sizes
+// dts[off[idx] .. off[idx + 1]) instead of the step-size control: the gather,
+// the RP integration and the aggregated void-fraction scatter of the
+// persistent kernel, one bubble per lane.  A bubble that has not reached the
+// sync time when its recorded sizes run out sets the sub-step cap flag.
+template <typename F>
+__global__ __launch_bounds__(256) void k_rp_replay(const RPLaunch *kp, int cur, int plm, int pln,
+                                                    int plp, const F *__restrict__ p0,
+                                                    int64_t *__restrict__ fg_target,
+                                                    StepScalars *sc, const int64_t *off,
+                                                    const double *dts) {
+    const RPLaunch &P = *kp;
+    const GridParams &g = P.g;
+    const BubbleParams &bp = P.bp;
+    const BubbleArrays &b = P.b;
+    const int lane = threadIdx.x & 63;
+    __shared__ WaveAgg agg_all[4];
+    __shared__ double vtab[kRpVSize];
+    WaveAgg &agg = agg_all[threadIdx.x >> 6];
+    agg_clear(agg, lane);
+    rp_vtab_stage(bp.rp_shared, vtab);
+    __syncthreads();
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxt = cur ^ 1;
+    bool bad = false, capped = false;
+    int cnt = 0;
+    unsigned int done = 0;
+    if (idx < b.n) {
+        const double x = b.x[idx], y = b.y[idx];
+        const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+        const double PL = gather_p0(g, bp, p0, x, y, ibnx, ibny);
+        b.PL[plp][idx] = PL;
+        const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
+        const double R0 = b.R0[idx], PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
+        const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
+                             bp.T0 / (PG0 * R03)};
+        RPLaneF<double> L;
+        rp_begin_g(L, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
+                   b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], PL, g.dt,
+                   bp.PL0);
+        const int64_t o = off[idx], n = off[idx + 1] - o;
+        for (int64_t k = 0; k < n && L.remain > 0.0; k++)
+            rp_substep_g<double, true>(L, fk, fs, dts[o + k]);
+        capped = L.remain > 0.0;
+        RPState out;
+        double R_t, Q_B;
+        rp_end_g(L, g.dt, &out, &R_t, &Q_B);
+        b.R[nxt][idx] = out.Rp;
+        b.Rn[nxt][idx] = out.Rn;
+        b.d1R[nxt][idx] = out.d1R;
+        b.PG[nxt][idx] = out.PG;
+        b.bdt[nxt][idx] = out.dt;
+        b.re[nxt][idx] = out.re;
+        b.R_t[idx] = R_t;
+        b.Q_B[idx] = Q_B;
+        b.sub[idx] = L.counter;
+        cnt = L.counter;
+        done = 1;
+        bad |= !isfinite(R_t);
+        scatter_bubble_agg(g, bp, x, y, ibnx, ibny, fg_temp_of(g, bp, b.n_B[idx], R_t, x), agg,
+                           fg_target, &bad);
+    }
+    agg_flush(agg, lane, fg_target);
+    unsigned long long sum = (unsigned long long)cnt;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        cnt = max(cnt, __shfl_xor(cnt, o2, 64));
+        sum += __shfl_xor(sum, o2, 64);
+        done += __shfl_xor(done, o2, 64);
+    }
+    if (lane == 0) {
+        if (done) atomicAdd(&sc->done, done);
+        if (sum) atomicAdd(&sc->sub_sum, sum);
+        atomicMax(&sc->sub_max, cnt);
+    }
+    if (bad) atomicOr(&sc->flags, 2);
+    if (capped) atomicOr(&sc->flags, 4);
+}
+
 // The initial void fraction (calculate_void_fraction at set-up,
 // bubble_CUDA.cu:180-185): scatter of R_t = R0.
 __global__ __launch_bounds__(256) void k_scatter_initial(GridParams g, BubbleParams bp,
@@ -578,6 +688,60 @@ __global__ __launch_bounds__(256) void k_scatter_radii(GridParams g, BubbleParam
     }
     agg_flush(agg, lane, fg);
     if (bad) atomicOr(&sc->flags, 2);
+}
+
+// The scatter of the current clamped radii with another accumulation than
+// the int64 fixed point, for comparing accumulation schemes (diagnostic):
+//   mode 1: FP64 atomic adds into out (double), the arithmetic of the
+//           reference's CAS loop (bubble_CUDA_kernel.cuh:95-114) -- the
+//           result depends on the order the adds land in;
+//   mode 2: compensated FP32: atomic adds into a float sum; the add returns
+//           the old value, so its rounding error is recovered exactly
+//           (TwoSum) and added, with the contribution's own rounding to
+//           FP32, into a second float array comp; the value is sum + comp;
+//   mode 3: plain FP32 atomic adds into the float sum.
+// out holds `frame` doubles (mode 1) or 2 * frame floats (modes 2, 3: sum,
+// then comp) and must be zero on entry.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_scatter_variant(GridParams g, BubbleParams bp,
+                                                         BubbleArrays b, void *out,
+                                                         int64_t frame) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= b.n) return;
+    const double x = b.x[idx], y = b.y[idx];
+    const int ibnx = bin_index(x, g.rdx, 0.5), ibny = bin_index(y, g.rdy, 0.5);
+    const double fg_temp = fg_temp_of(g, bp, b.n_B[idx], b.R_t[idx], x);
+    const int nb = bp.mbe - bp.mbs + 1;
+    for (int a = 0; a < nb; a++) {
+        const int i = ibnx + bp.mbs + a;
+        const double wx = delta_w(bp.dcx, bp.dcy, i, g.dx, x);
+        for (int c = 0; c < nb; c++) {
+            const int j = ibny + bp.mbs + c;
+            if (i < 0 || i > g.N + 1 || j < 0 || j > g.M + 1) continue;
+            double v;
+            {
+#pragma clang fp contract(off)
+                v = fg_temp * wx * delta_w(bp.dcx, bp.dcz, j, g.dy, y);
+            }
+            const int64_t o = frame_off(i, j, g.pitch);
+            if (MODE == 1) {
+                atomicAdd((double *)out + o, v);
+            } else {
+                float *sum = (float *)out, *comp = (float *)out + frame;
+                const float vf = (float)v;
+                const float old = atomicAdd(sum + o, vf);
+                if (MODE == 2) {
+#pragma clang fp contract(off)
+                    const float s = old + vf;  // the value the atomic stored
+                    const float bv = s - old;
+                    const float av = s - bv;
+                    const float err = (old - av) + (vf - bv);
+                    const float ev = (float)(v - (double)vf);
+                    atomicAdd(comp + o, err + ev);
+                }
+            }
+        }
+    }
 }
 
 // VoidFractionCylinderKernel (bubble_CUDA_kernel.cuh:559-573) for order 2:
@@ -867,6 +1031,40 @@ void launch_scatter_radii(hipStream_t s, const GridParams &g, const BubbleParams
     hipLaunchKernelGGL(k_scatter_radii, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, Rt, fg,
                        sc);
 }
+
+void launch_scatter_variant(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                            const BubbleArrays &b, int mode, void *out, int64_t frame) {
+    if (b.n <= 0) return;
+    const dim3 grid((b.n + 255) / 256), blk(256);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_scatter_variant<1>, grid, blk, 0, s, g, bp, b, out, frame);
+    else if (mode == 2)
+        hipLaunchKernelGGL(k_scatter_variant<2>, grid, blk, 0, s, g, bp, b, out, frame);
+    else
+        hipLaunchKernelGGL(k_scatter_variant<3>, grid, blk, 0, s, g, bp, b, out, frame);
+}
+
+void launch_rp_record(hipStream_t s, const RPLaunch *dev, int n, int cur, int plm, int pln,
+                      int plp, const int64_t *off, double *dts, unsigned int *mismatch) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_rp_record, dim3((n + 255) / 256), dim3(256), 0, s, dev, cur, plm, pln,
+                       plp, off, dts, mismatch);
+}
+
+template <typename F>
+void launch_rp_replay(hipStream_t s, const RPLaunch *dev, int n, int cur, int plm, int pln,
+                      int plp, const F *p0, int64_t *fg_target, StepScalars *sc,
+                      const int64_t *off, const double *dts) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_rp_replay<F>, dim3((n + 255) / 256), dim3(256), 0, s, dev, cur, plm, pln,
+                       plp, p0, fg_target, sc, off, dts);
+}
+template void launch_rp_replay<double>(hipStream_t, const RPLaunch *, int, int, int, int, int,
+                                       const double *, int64_t *, StepScalars *, const int64_t *,
+                                       const double *);
+template void launch_rp_replay<float>(hipStream_t, const RPLaunch *, int, int, int, int, int,
+                                      const float *, int64_t *, StepScalars *, const int64_t *,
+                                      const double *);
 
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg) {
     hipLaunchKernelGGL(k_cyl_fold, dim3((g.M + 2 + 255) / 256), dim3(256), 0, s, g, fg);

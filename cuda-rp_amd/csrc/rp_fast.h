@@ -277,8 +277,11 @@ __device__ inline void rp_begin_g(RPLaneF<T> &L, T Rp, T Rn, T d1R, T PG, double
 }
 
 // One pass of the sub-step loop body (bubble_CUDA_kernel.cuh:508-538).
-template <typename T>
-__device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const RPFastShared &s) {
+// FORCED: the step size is forced_dtp instead of the step-size control's
+// (the replay of a recorded schedule; the rest of the sub-step is unchanged).
+template <typename T, bool FORCED = false>
+__device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const RPFastShared &s,
+                                    double forced_dtp = 0.0) {
     // solveRayleighPlesset (bubble_CUDA_kernel.cuh:227-260)
     const T PGn = L.PG;
     const double t = L.time;
@@ -294,6 +297,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     const double ad = fabs(val(d1R)), aa = fabs(val(Rnd2R));
     if (ad > kDblEps) dtp = fmin(dtp, (0.01 * val(Rn)) * drcp(ad));
     if (aa > kDblEps) dtp = fmin(dtp, (0.1 * val(Rn)) * drsqrt(aa));
+    if constexpr (FORCED) dtp = forced_dtp;
     const double idtm = drcp(dtm);
     const double r = dtp * idtm;
     const T acc = (dtp * Rnd2R) * iRn;  // (dtp / Rn) Rnd2R

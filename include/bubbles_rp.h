@@ -245,6 +245,38 @@ int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld);
  * brp_set_bubbles) into dst. */
 int brp_get_bubbles(brp_handle *h, int32_t which, double *dst);
 
+/* Frozen schedule, for finite differences of the piecewise-smooth map that
+ * brp_adjoint differentiates: mode 1 records every step's Picard count and,
+ * for every RP launch, every bubble's accepted RP step sizes; mode 2 replays
+ * them in order instead of the residual test (bubble_CUDA.cu:243) and the
+ * step-size control (kernel.cuh:241-247) -- the rest of the arithmetic is
+ * unchanged; mode 0 is the reference behaviour.  Setting a mode rewinds the
+ * replay; mode 1 starts a new record.  Shortened-chain RP (rp_exact = 0)
+ * only; brp_adjoint needs mode 0.  brp_schedule_copy makes dst (same device,
+ * same bubble count) replay src's record, which the two then share.  No
+ * reference counterpart (the reference has no adjoint). */
+int brp_schedule_mode(brp_handle *h, int32_t mode);
+int brp_schedule_copy(brp_handle *dst, const brp_handle *src);
+
+/* Accumulation schemes of the void-fraction scatter (brp_scatter_variant). */
+enum {
+    BRP_VF_INT64 = 0,           /* the library's: int64 fixed point (value * 2^60),
+                                   exact integer sums, independent of order  */
+    BRP_VF_FP64_ATOMIC = 1,     /* FP64 atomic adds, the reference's arithmetic
+                                   (atomicAdd by CAS, kernel.cuh:95-114); the
+                                   rounding depends on the order of the adds */
+    BRP_VF_FP32_COMPENSATED = 2,/* FP32 atomic adds plus each add's exact
+                                   rounding error summed in a second FP32 array */
+    BRP_VF_FP32_ATOMIC = 3      /* plain FP32 atomic adds                    */
+};
+
+/* Diagnostic: the void-fraction scatter (VoidFractionReverseLookupKernel,
+ * kernel.cuh:576-619) of the bubbles' current clamped radii R_t at their
+ * current positions, accumulated with scheme `mode`, copied as doubles into
+ * dst (the BRP_FIELD_FG layout, leading dimension ld).  The simulation state
+ * is not changed.  Used to compare accumulation schemes on one population. */
+int brp_scatter_variant(brp_handle *h, int32_t mode, double *dst, int64_t ld);
+
 /* Overwrites one bubble array (n_bubbles host doubles in the order given to
  * brp_set_bubbles): X, Y, R_T, R_P, R_N, D1R, PG, PL_P, PL_N, PL_M, Q_B, DT,
  * RE, VX or VY -- the state a step starts from (bubble_t,

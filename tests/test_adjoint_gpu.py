@@ -158,3 +158,118 @@ def test_autograd_function(brp):
     np.testing.assert_allclose(P.grad.numpy(), 2.0 * r["grad_phase"], rtol=1e-12)
     np.testing.assert_allclose(fg0.grad.numpy(), 2.0 * r["grad_fg0"], rtol=1e-12,
                                atol=1e-12 * np.abs(r["grad_fg0"]).max())
+
+
+def test_adjoint_refuses_exact_rp_and_schedule_modes(brp):
+    """brp_adjoint linearises the shortened-chain sub-step; with rp_exact = 1
+    (another sub-step count on a few percent of bubbles) it refuses instead of
+    returning the gradient of another trajectory; likewise while a frozen
+    schedule is being recorded or replayed."""
+    cfg, bub, src = cases.adjoint_case(nb=24)
+    cfg.rp_exact = 1
+    g = cases.make(brp.Simulation, cfg, bub, src)
+    with pytest.raises(brp.BrpError) as e:
+        g.adjoint(cases.focal_box(cfg), STEPS, WINDOW, 20)
+    assert e.value.code == brp.abi.ERR_INVALID and "rp_exact" in str(e.value)
+    cfg.rp_exact = 0
+    g = cases.make(brp.Simulation, cfg, bub, src)
+    g.schedule_mode(1)
+    with pytest.raises(brp.BrpError) as e:
+        g.adjoint(cases.focal_box(cfg), STEPS, WINDOW, 20)
+    assert e.value.code == brp.abi.ERR_STATE
+
+
+def _gpu_frozen_J(brp, cfg, bub, src, base, fg_map=None, steps=STEPS, window=WINDOW, box=None):
+    s = cases.make(brp.Simulation, cfg, bub, src)
+    if fg_map is not None:
+        s.set_field("fg", fg_map)
+    s.schedule_from(base)
+    s.schedule_mode(2)
+    J = cases.focal_objective(s, steps, box or cases.focal_box(cfg), window)
+    s.close()
+    return J
+
+
+def test_gpu_frozen_schedule(brp):
+    """The GPU's frozen-schedule mode: replaying its own record at the same
+    parameters reproduces the recorded run bit for bit, and at perturbed
+    source parameters its J agrees with the oracle's own frozen-schedule J
+    (each replaying the schedule it recorded at the base point; FP64)."""
+    cfg, bub, (amp, phase) = cases.adjoint_case()
+    box = cases.focal_box(cfg)
+    base = cases.make(brp.Simulation, cfg, bub, (amp, phase))
+    base.schedule_mode(1)
+    J0 = cases.focal_objective(base, STEPS, box, WINDOW)
+    rec, J0o = _oracle_J(cfg, bub, (amp, phase), mode=1)
+    assert J0 == pytest.approx(J0o, rel=1e-9)
+    again = cases.make(brp.Simulation, cfg, bub, (amp, phase))
+    again.schedule_from(base)
+    again.schedule_mode(2)
+    assert cases.focal_objective(again, STEPS, box, WINDOW) == J0
+    for n in ("p0", "fg"):
+        np.testing.assert_array_equal(again.field(n), base.field(n))
+    np.testing.assert_array_equal(again.bubbles("R_t"), base.bubbles("R_t"))
+    np.testing.assert_array_equal(again.bubbles("sub"), base.bubbles("sub"))
+    for e in (0, 5):
+        a = amp.copy()
+        a[e] *= 1.0 + 1e-3
+        Jg = _gpu_frozen_J(brp, cfg, bub, (a, phase), base)
+        Jo = _oracle_J(cfg, bub, (a, phase), rec)[1]
+        print(f"element {e}: frozen J GPU {Jg:.12e} oracle {Jo:.12e}", flush=True)
+        assert Jg == pytest.approx(Jo, rel=1e-9)
+
+
+def test_c4_adjoint_against_frozen_differences(brp):
+    """The adjoint workload at its own size (1024^2, 65536 bubbles with
+    R0 ~ U(1, 5) um, 64 elements with amp ~ U(0.3, 0.7) MPa, phase ~ U(0, 2 pi),
+    the 2 x 2 mm objective box), FP64: brp_adjoint against the GPU's own
+    frozen-schedule central differences (h = 1e-4 relative) on 8 seeded
+    parameters (3 amplitudes, 3 phases, 2 cells of the initial void-fraction
+    map), agreement 1e-3.  The horizon is 400 steps with J over the last 100:
+    the wavefront reaches the bubbles near step 300, and over the 4000-step
+    window of the benchmark the cavitating cloud is chaotic (the gradient
+    norm grows to ~1e19), so a perturbation of 1e-4 leaves the linear range
+    and no finite difference can check the derivative there."""
+    cfg, bub, src, box = cases.c4_case(precision=brp.abi.PREC_FP64)
+    amp, phase = src
+    steps, window = 400, 100
+    base = cases.make(brp.Simulation, cfg, bub, src)
+    base.schedule_mode(1)
+    J0 = cases.focal_objective(base, steps, box, window)
+    fg_base = cases.make(brp.Simulation, cfg, bub, src).field("fg")
+    g = cases.make(brp.Simulation, cfg, bub, src)
+    r = g.adjoint(box, steps, window, 50)
+    assert r["J"] == pytest.approx(J0, rel=1e-12)
+    assert r["schedule_mismatch"] == 0
+    rng = cases.rng(44)
+    picks = [("amp", int(k)) for k in rng.choice(64, 3, replace=False)] + \
+            [("phase", int(k)) for k in rng.choice(64, 3, replace=False)]
+    nz = np.argwhere(fg_base > 0.1 * fg_base.max())  # cells inside the cloud
+    picks += [("fg0", tuple(nz[int(k)])) for k in rng.choice(len(nz), 2, replace=False)]
+    worst = 0.0
+    for which, k in picks:
+        Jpm = []
+        for sgn in (1, -1):
+            a, p, m = amp.copy(), phase.copy(), None
+            if which == "amp":
+                h = 1e-4 * amp[k]
+                a[k] += sgn * h
+            elif which == "phase":
+                h = 1e-4
+                p[k] += sgn * h
+            else:
+                # the map is stored as int64 multiples of 2^-60: difference
+                # over the perturbation actually applied
+                q = lambda v: float(np.rint(v * 2.0 ** 60)) * 2.0 ** -60
+                h = (q(fg_base[k] * (1 + 1e-4)) - q(fg_base[k] * (1 - 1e-4))) / 2
+                m = fg_base.copy()
+                m[k] = q(fg_base[k] * (1 + sgn * 1e-4))
+            Jpm.append(_gpu_frozen_J(brp, cfg, bub, (a, p), base, fg_map=m, steps=steps,
+                                     window=window, box=box))
+        fd = (Jpm[0] - Jpm[1]) / (2 * h)
+        got = r["grad_" + which][k]
+        err = abs(got - fd) / max(abs(fd), 1e-30)
+        worst = max(worst, err)
+        print(f"{which} {k}: adjoint {got:.8e} FD {fd:.8e} rel err {err:.2e}", flush=True)
+        assert err < 1e-3, (which, k, got, fd)
+    print(f"C4 adjoint vs frozen-schedule differences: worst relative error {worst:.2e}")

@@ -2,8 +2,11 @@
 
 C1 runs in full (256^2, 1024 bubbles, 2000 steps, 64-element aperture) in
 FP64 (shortened-chain and operation-by-operation RP) and mixed precision:
-per-step Picard counts, p0 / vx / vy / f_g every 250 steps, every bubble's
-R_t history (every step) and, in FP64, bit-exact bubble bins.
+per-step Picard counts, focal-box pressure and every bubble's R_t, the
+fields every 50 steps and, in FP64, bit-exact bubble bins.  The cavitating
+cloud is chaotic, so the records are compared pointwise up to the horizon
+within which two runs that round differently can agree, and by their
+statistics after it.
 
 C2 (2048^2, 2^20 bubbles) and C3 (4096^2, 2^22 clustered bubbles) are too
 large for the oracle to run from rest inside a test, so the GPU runs them to
@@ -12,8 +15,7 @@ the onset of cavitation and the oracle is seeded with the GPU's state there
 the same steps and are compared on p0, f_g, the R_t of a fixed random subset
 of 4096 bubbles and the Picard counts.
 
-Tolerances (relative L2 over the field or over the bubbles, per record) are
-stated with each test; the reasons are in the test docstrings.
+Tolerances are stated with each test, with their reasons.
 """
 import os
 import time
@@ -29,23 +31,31 @@ pytestmark = pytest.mark.gpu
 
 OMP = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
 C1_STEPS = 2000
-C1_SAVE = 250
+C1_SAVE = 50
+C1_HORIZON = 500  # pointwise comparison up to here (see test_c1_full_against_oracle)
 FIELDS = ("p0", "vx", "vy", "fg")
 
 
 def _c1_run(sim, steps=C1_STEPS):
-    """Steps one at a time: Picard counts, fields every C1_SAVE steps, R_t of
-    every bubble after every step, bins at the saves."""
-    npic, saves, bins = [], {}, {}
+    """Steps one at a time: Picard counts, the focal-box mean of p0^2 and
+    every bubble's R_t after every step, fields every C1_SAVE steps and bins
+    at the saves up to the horizon."""
+    cfg = sim.cfg
+    i0, i1, j0, j1 = cases.focal_box(cfg, 10)  # 2 x 2 mm at the focus
+    npic, fp2, saves, bins = [], [], {}, {}
     rt = np.zeros((steps, sim.info()["n_bubbles"]))
     for s in range(1, steps + 1):
         st = sim.step(1)
         npic.append(st[0]["n_pic"])
+        p = sim.field("p0")
+        fp2.append(float(np.mean(p[j0:j1 + 1, i0:i1 + 1] ** 2)))
         rt[s - 1] = sim.bubbles("R_t")
         if s % C1_SAVE == 0:
             saves[s] = {n: sim.field(n) for n in FIELDS}
-            bins[s] = {n: sim.bubbles(n) for n in ("ibm_x", "ibm_y", "ibn_x", "ibn_y")}
-    return np.array(npic), saves, rt, bins
+            if s <= C1_HORIZON:
+                bins[s] = {n: sim.bubbles(n) for n in ("ibm_x", "ibm_y", "ibn_x", "ibn_y")}
+    return {"npic": np.array(npic), "fp2": np.array(fp2), "saves": saves, "rt": rt,
+            "bins": bins, "R0": sim.bubbles("R0")}
 
 
 @pytest.fixture(scope="module")
@@ -63,26 +73,43 @@ def c1_oracle():
     return out
 
 
-def _history_errors(g, o):
-    """Per saved step: field rel L2 / L-inf; per step: R_t rel L2 over bubbles."""
-    gp, gsaves, grt, gbins = g
-    op, osaves, ort, obins = o
-    fields = {s: {n: (rel_l2(gsaves[s][n], osaves[s][n]), rel_linf(gsaves[s][n], osaves[s][n]))
-                  for n in FIELDS} for s in sorted(osaves)}
-    rt = np.array([rel_l2(grt[k], ort[k]) for k in range(len(ort))])
-    return fields, rt
+def c1_statistics(r, start=C1_HORIZON):
+    """What a user of the run reads from steps start+1 .. end: the time mean
+    of the focal-box p0^2 (the heat-deposition proxy the adjoint
+    differentiates), the distribution of R_t / R0 over bubbles and steps, the
+    fraction of bubble-steps beyond 2 R0 and the mean Picard count."""
+    w = slice(start, len(r["npic"]))
+    ratio = r["rt"][w] / r["R0"]
+    q = np.percentile(ratio, (50, 90, 99, 99.9))
+    return {"focal_p2": r["fp2"][w].mean(), "p50": q[0], "p90": q[1], "p99": q[2],
+            "p99.9": q[3], "frac_2R0": float(np.mean(ratio > 2.0)),
+            "picard": float(r["npic"][w].mean())}
+
+
+# Bounds on the statistics of steps 501-2000, relative unless stated: about
+# 2.5 times the spread between two oracle runs whose only difference is one
+# ulp of one bubble's R0 (measured: focal p^2 4.2 %, percentiles 0.8-1.1 %
+# and 5.5 % at 99.9, fraction beyond 2 R0 0.7 points, Picard 0.2 %).
+C1_STAT_TOL = {"focal_p2": 0.12, "p50": 0.05, "p90": 0.05, "p99": 0.05, "p99.9": 0.15,
+               "frac_2R0": 0.03, "picard": 0.05}
 
 
 @pytest.mark.parametrize("prec,exact", [(0, 0), (0, 1), (1, 0)])
 def test_c1_full_against_oracle(brp, c1_oracle, prec, exact):
-    """C1 as stated (2000 steps; the wavefront reaches the bubbles near step
-    300 and they cavitate, up to ~3000 RP sub-steps per step).  FP64: the
-    bubbles' inertial collapses amplify last-ulp differences between the
-    GPU's and the host's sqrt / reciprocal / transcendental roundings, so the
-    bound is 1e-7 on every record (measured values are printed).  Mixed: the
-    FP32 fields move the collapse times; the bound is 1e-2 on p0 and 5e-2 on
-    R_t per record (pointwise; collapse timing differences of one sub-step
-    show as O(1) radius differences of single bubbles)."""
+    """C1 as stated: 256^2, 1024 bubbles, 64-element aperture, 2000 steps.
+    The wavefront reaches the bubbles near step 300 and they cavitate
+    (~3000 RP sub-steps in the longest chains).  The cloud is chaotic: in the
+    oracle itself, one ulp of one bubble's R0 grows to a relative difference
+    of 3e-9 by step 500, 1e-2 by step 750 and O(1) by step 1000
+    (tests/test_oracle_pins_fields.py::test_c1_cloud_is_chaotic), so two runs
+    that round differently can agree pointwise only up to that horizon.
+    Checked:
+      * FP64 (shortened-chain and operation-by-operation RP): every record
+        (p0, vx, vy, f_g every 50 steps; every bubble's R_t every step) within
+        1e-7 relative L2 and identical Picard counts and bubble bins up to
+        step 500; mixed: within 1e-4 up to step 300 (the first cavitation);
+      * over steps 501-2000, the statistics of c1_statistics within
+        C1_STAT_TOL of the oracle's."""
     cfg, bub, src = cases.c1_case(precision=prec)
     cfg.rp_exact = exact
     sim = cases.make(brp.Simulation, cfg, bub, src)
@@ -90,31 +117,37 @@ def test_c1_full_against_oracle(brp, c1_oracle, prec, exact):
     g = _c1_run(sim)
     print(f"C1 GPU prec={prec} exact={exact}: {time.perf_counter() - t0:.1f} s", flush=True)
     sim.close()
-    fields, rt = _history_errors(g, c1_oracle)
-    for s, e in fields.items():
-        print(f"  step {s}: " + ", ".join(f"{n} L2 {a:.2e} Linf {b:.2e}" for n, (a, b) in e.items()),
-              flush=True)
-    print(f"  R_t rel L2 by step: max {rt.max():.2e} at step {int(rt.argmax()) + 1}, "
-          f"at 500/1000/1500/2000: {rt[499]:.2e} {rt[999]:.2e} {rt[1499]:.2e} {rt[1999]:.2e}",
+    o = c1_oracle
+    worst = {}
+    for st in sorted(o["saves"]):
+        e = {n: rel_l2(g["saves"][st][n], o["saves"][st][n]) for n in FIELDS}
+        worst[st] = max(e.values())
+        if st % 250 == 0:
+            print(f"  step {st}: " + ", ".join(f"{n} {v:.2e}" for n, v in e.items()), flush=True)
+    rt = np.array([rel_l2(g["rt"][k], o["rt"][k]) for k in range(C1_STEPS)])
+    print(f"  R_t rel L2 at steps 300/400/500/750/1000: {rt[299]:.2e} {rt[399]:.2e} "
+          f"{rt[499]:.2e} {rt[749]:.2e} {rt[999]:.2e}", flush=True)
+    same = g["npic"] == o["npic"]
+    print(f"  Picard counts equal on {int(same.sum())} of {C1_STEPS} steps, the first "
+          f"difference at step {int(np.argmin(same)) + 1 if not same.all() else None}",
           flush=True)
-    gp, op = g[0], c1_oracle[0]
-    print(f"  Picard mismatching steps: {int(np.sum(gp != op))} of {len(op)}, "
-          f"oracle mean {op.mean():.2f}", flush=True)
-    assert op[300:].mean() > 2.0  # the bubbles are driven: Picard iterations happen
+    sg, so = c1_statistics(g), c1_statistics(o)
+    for k in sg:
+        print(f"  steps 501-2000 {k}: GPU {sg[k]:.6g} oracle {so[k]:.6g}", flush=True)
+    assert o["npic"][300:].mean() > 2.0 and o["rt"].max() / o["R0"].max() > 2.0  # cavitating
+    horizon, tol = (C1_HORIZON, 1e-7) if prec == 0 else (300, 1e-4)
+    assert max(v for st, v in worst.items() if st <= horizon) < tol
+    assert rt[:horizon].max() < tol
     if prec == 0:
-        np.testing.assert_array_equal(gp, op)
-        for s in g[3]:
-            for n in g[3][s]:
-                np.testing.assert_array_equal(g[3][s][n], c1_oracle[3][s][n], err_msg=f"{n} {s}")
-        for s, e in fields.items():
-            for n, (l2, _) in e.items():
-                assert l2 < 1e-7, (s, n, l2)
-        assert rt.max() < 1e-7
-    else:
-        assert np.sum(gp != op) <= 0.01 * len(op)
-        for s, e in fields.items():
-            assert e["p0"][0] < 1e-2, (s, e["p0"])
-        assert rt.max() < 5e-2
+        np.testing.assert_array_equal(g["npic"][:horizon], o["npic"][:horizon])
+        for st in g["bins"]:
+            for n in g["bins"][st]:
+                np.testing.assert_array_equal(g["bins"][st][n], o["bins"][st][n])
+    for k, t in C1_STAT_TOL.items():
+        if k == "frac_2R0":
+            assert abs(sg[k] - so[k]) < t, k
+        else:
+            assert abs(sg[k] / so[k] - 1.0) < t, k
 
 
 def _seeded_pair(brp, cfg, bub, pre_steps, steps, subset, seed_from=None):
@@ -153,8 +186,8 @@ def _seeded_pair(brp, cfg, bub, pre_steps, steps, subset, seed_from=None):
     return rows
 
 
-C2_STEPS = int(os.environ.get("BRP_C2_STEPS", "3"))
-C3_STEPS = int(os.environ.get("BRP_C3_STEPS", "2"))
+C2_STEPS = int(os.environ.get("BRP_C2_STEPS", "8"))
+C3_STEPS = int(os.environ.get("BRP_C3_STEPS", "4"))
 
 
 @pytest.mark.parametrize("prec", [0, 1])

@@ -279,28 +279,43 @@ def test_error_paths(brp):
         brp.Simulation(bad)
 
 
-@pytest.mark.parametrize("where", ["p0", "R"])
-def test_nonfinite_is_reported(brp, where):
-    """BRP_ERR_NONFINITE: a NaN planted in the pressure at the bubble box, or
-    a bubble radius set to zero (its RP sub-step then divides by zero), is
-    reported by the step that meets it; every kernel still ends normally
+def test_nonfinite_pressure_is_reported(brp):
+    """BRP_ERR_NONFINITE: a NaN planted in the pressure at the bubble box
+    reaches the velocities, the pressures and, through the gather, the RP
+    inputs; the step that meets it reports it and every kernel still ends
     (the sub-step sizes stay finite, so each RP integration ends)."""
     cfg, b = cases.bubbly_case(48, nb=50)
     g = cases.make(brp.Simulation, cfg, b)
     g.step(5)
-    if where == "p0":
-        p = g.field("p0")
-        ny, nx = p.shape
-        p[ny // 2, nx // 2] = np.nan
-        g.set_field("p0", p)
-    else:
-        R = g.bubbles("R_p")
-        R[7] = 0.0
-        g.set_bubble_array("R_p", R)
+    p = g.field("p0")
+    ny, nx = p.shape
+    p[ny // 2, nx // 2] = np.nan
+    g.set_field("p0", p)
     with pytest.raises(brp.BrpError) as e:
         g.step(20)
     assert e.value.code == brp.abi.ERR_NONFINITE, str(e.value)
     g.close()
+
+
+def test_zero_radius_hits_the_substep_cap_like_the_oracle(brp):
+    """A bubble radius set to 0: the step-size rule 0.01 R / |dR/dt|
+    (kernel.cuh:245-246) gives a sub-step of 0, so the integration can never
+    reach the sync time; GPU and oracle both stop at max_substeps and report
+    BRP_ERR_SUBSTEP_CAP (the GPU also flags the non-finite radius it wrote)."""
+    cfg, b = cases.bubbly_case(48, nb=50, amp_pa=1.5e6)
+    cfg.max_substeps = 3000
+    codes = []
+    for cls in (brp.Simulation, ol.OracleSim):
+        s = cases.make(cls, cfg, b)
+        s.step(100)  # the wavefront has reached the bubbles: dR/dt != 0
+        R = s.bubbles("R_p")
+        R[7] = 0.0
+        s.set_bubble_array("R_p", R)
+        with pytest.raises(brp.BrpError) as e:
+            s.step(5)
+        codes.append(e.value.code)
+        s.close()
+    assert codes == [brp.abi.ERR_SUBSTEP_CAP] * 2
 
 
 def test_standing_wave_closed_form(brp):
@@ -366,3 +381,47 @@ def test_temperature(brp, case, prec, tol):
     assert err < tol
     if prec == 0:  # the temperature is one-way: the acoustic fields are unchanged
         assert rel_l2(g.field("p0"), o.field("p0")) < (1e-12 if b is None else 1e-9)
+
+
+def test_void_fraction_accumulation_schemes(brp):
+    """The void-fraction scatter of one population (a 6000-bubble cluster in
+    2 x 2 cells, thousands of contributions per cell, plus a lattice) with
+    the four accumulation schemes of brp_scatter_variant, against the
+    oracle's serial FP64 sum of the same contributions: the library's int64
+    fixed point is exact to its 2^-60 resolution (order-free), FP64 atomics
+    to FP64 rounding (order-dependent), compensated FP32 to FP32 rounding of
+    the final value, plain FP32 atomics only to the FP32 rounding of every
+    add."""
+    cfg, _ = cases.bubbly_case(128, nb=10, amp_pa=0.3e6)
+    fp_x = cfg.ext_focus_x
+    fp_y = cfg.tr_focal_dist * 0.5 * np.sqrt(3.0)
+    r = np.random.default_rng(12)
+    xc = fp_x + (r.random(6000) - 0.5) * 2e-4
+    yc = fp_y + (r.random(6000) - 0.5) * 2e-4
+    gx, gy = np.meshgrid(np.arange(40) * 2e-4, np.arange(40) * 2e-4)
+    x = np.concatenate([xc, fp_x - 4e-3 + gx.ravel() + 0.3e-4])
+    y = np.concatenate([yc, fp_y - 4e-3 + gy.ravel() + 0.3e-4])
+    R0 = 1e-6 + 4e-6 * r.random(len(x))
+    bub = (x, y, R0, None, 1e-9)
+    g = cases.make(brp.Simulation, cfg, bub)
+    st = g.step(180)  # the wavefront has reached the bubbles
+    assert st[-1]["n_pic"] > 0
+    o = cases.make(ol.OracleSim, cfg, bub)
+    for n in ("x", "y", "R_t"):
+        o.set_bubble_array(n, g.bubbles(n))
+    dx = cfg.grid_lx / cfg.grid_x
+    q = 4.0 / 3.0 * np.pi * o.bubbles("n_B") * g.bubbles("R_t") ** 3 / (dx * dx)
+    ref = o.scatter_values(q)
+    errs = {}
+    for mode, tol in ((brp.abi.VF_INT64, 1e-9), (brp.abi.VF_FP64_ATOMIC, 1e-14),
+                      (brp.abi.VF_FP32_COMPENSATED, 1e-7), (brp.abi.VF_FP32_ATOMIC, 1e-3)):
+        v = g.scatter_variant(mode)
+        errs[mode] = (rel_l2(v, ref), rel_linf(v, ref))
+        assert errs[mode][0] < tol, (mode, errs[mode])
+    print("accumulation schemes vs serial FP64 (rel L2, rel Linf):",
+          {k: f"{a:.2e}/{b:.2e}" for k, (a, b) in errs.items()}, flush=True)
+    assert errs[brp.abi.VF_FP32_COMPENSATED][0] < 0.1 * errs[brp.abi.VF_FP32_ATOMIC][0]
+    # the product's own f_g is the int64 scheme of the last scatter
+    np.testing.assert_array_equal(g.scatter_variant(brp.abi.VF_INT64), g.field("fg"))
+    g.close()
+    o.close()

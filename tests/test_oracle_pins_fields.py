@@ -442,3 +442,36 @@ def test_bubble_heat_is_zero_at_equilibrium():
                                              -2e5])
         assert abs(driven[7]) > 0
         assert abs(rest[7]) < 1e-9 * abs(driven[7]), (rest[7], driven[7])
+
+
+# ---- the predictability horizon of a cavitating cloud ------------------------
+def test_c1_cloud_is_chaotic():
+    """The C1 workload (256^2, 1024 bubbles, 0.5 MPa) run twice by the oracle,
+    the second time with one bubble's R0 moved by one ulp: the runs agree to
+    rounding until the wavefront reaches the bubbles (~step 300), to ~1e-9 at
+    step 500, and differ by more than 1e-4 at step 750 -- the bubbles'
+    collapses amplify any rounding difference by about 1e6 per 250 steps, so
+    two correct implementations that round differently agree pointwise only
+    up to about step 500 of this run."""
+    out = []
+    for perturb in (False, True):
+        cfg, (x, y, R0, _, fg_in), src = cases.c1_case()
+        R0 = R0.copy()
+        if perturb:
+            R0[0] = np.nextafter(R0[0], 1.0)
+        o = ol.OracleSim(cfg, threads=8)
+        o.set_bubbles(x, y, R0, None, fg_in)
+        o.set_source(*src)
+        o.prepare()
+        rec = {}
+        for s in (250, 500, 750):
+            o.step(s - (max(rec) if rec else 0))
+            rec[s] = (o.field("p0"), o.bubbles("R_t"))
+        out.append(rec)
+        o.close()
+    d = {s: (cases.rel_l2(out[1][s][0], out[0][s][0]), cases.rel_l2(out[1][s][1], out[0][s][1]))
+         for s in out[0]}
+    print({s: f"p0 {a:.1e} R_t {b:.1e}" for s, (a, b) in d.items()})
+    assert max(d[250]) < 1e-12
+    assert max(d[500]) < 1e-6
+    assert min(d[750]) > 1e-4
