@@ -62,6 +62,7 @@ def load_library():
         "brp_scatter_variant": (C.c_int, [vp, i32, dp, i64]),
         "brp_schedule_mode": (C.c_int, [vp, i32]),
         "brp_schedule_copy": (C.c_int, [vp, vp]),
+        "brp_get_bubbles_noalloc": (C.c_int, [vp, i32, dp]),
         "brp_get_clock": (C.c_int, [vp, C.POINTER(i64), dp, dp]),
         "brp_set_clock": (C.c_int, [vp, i64, C.c_double, C.c_double]),
         "brp_adjoint": (C.c_int, [vp, C.POINTER(abi.Objective), dp, dp, dp, i64,

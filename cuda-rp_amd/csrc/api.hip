@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -23,7 +24,57 @@
 
 using namespace brp;
 
+// Staging array of brp_get_bubbles_noalloc, for callers that cannot allocate
+// device memory: one bubble array of up to kScratchBubbles doubles (32 MiB),
+// resident on every GPU the library runs on until the process ends.  Host
+// code never passes `scratch` itself (in host code the name is a host-side
+// stand-in, not a device address): brp_create looks up each device's address
+// once, with that device current, and every use goes through it.
+constexpr int kScratchBubbles = 1 << 22;
+static __device__ __align__(256) double scratch[kScratchBubbles];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The address of `scratch` on the current device `dev` (looked up once per
+// device), or nullptr with the HIP error in *e.
+double *scratch_address(int dev, hipError_t *e) {
+    *e = hipSuccess;
+    if (dev < 0 || dev >= kMaxDevices) {
+        *e = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        *e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*e != hipSuccess) return nullptr;
+        g_scratch_dev[dev] = (double *)p;
+    }
+    return g_scratch_dev[dev];
+}
+
+// out[perm[k]] = the bubble value of sorted bubble k: a bubble array
+// (src != nullptr), or ibm / ibn (which_index 0..3) from the positions.
+__global__ void k_bubbles_to_caller(int n, const int *perm, const double *src, const int *isrc,
+                                    const double *x, const double *y, double rdx, double rdy,
+                                    int which_index, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double v;
+    if (src)
+        v = src[k];
+    else if (isrc)
+        v = (double)isrc[k];
+    else if (which_index & 1)
+        v = (double)bin_index(y[k], rdy, which_index < 2 ? 1.0 : 0.5);
+    else
+        v = (double)bin_index(x[k], rdx, which_index < 2 ? 1.0 : 0.5);
+    out[perm[k]] = v;
+}
 
 thread_local std::string g_create_error;
 
@@ -159,6 +210,9 @@ struct brp_handle {
     void *ax = nullptr, *bx = nullptr, *ay = nullptr, *by = nullptr;
     double *xu_d = nullptr, *rxp_d = nullptr;
     double *el_amp_d = nullptr, *el_phase_d = nullptr;
+    int *perm_d = nullptr;           // perm on the device (sorted -> caller index)
+    double *scratch_d = nullptr;     // this device's `scratch` (nullptr: lookup failed)
+    std::string scratch_err;
     double *T_d = nullptr, *dT_d = nullptr, *heat_d = nullptr;  // temperature solve
     int n_elem = 0;
 
@@ -421,6 +475,9 @@ int place_bubbles(brp_handle *h, const std::vector<double> &x, const std::vector
         HIPCHK(h, hipMemcpy(d, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         return BRP_OK;
     };
+    if ((rc = dalloc(h, (void **)&h->perm_d, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
+    if (n > 0)
+        HIPCHK(h, hipMemcpy(h->perm_d, h->perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     if ((rc = up(h->b.x, hx)) || (rc = up(h->b.y, hy)) || (rc = up(h->b.R0, R0)) ||
         (rc = up(h->b.PG0, PG0)) || (rc = up(h->b.calpha, ca)) || (rc = up(h->b.n_B, nB)) ||
         (rc = up(h->b.R_t, R0)) || (rc = up(h->b.Q_B, zero)) || (rc = up(h->b.vx, zero)) ||
@@ -1284,6 +1341,12 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     if (hipGetDevice(&h->device) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipGetDevice failed"));
     rp_grid_query(h->device, &h->cus, &h->per_cu);
+    {
+        hipError_t e;
+        h->scratch_d = scratch_address(h->device, &e);
+        if (!h->scratch_d)
+            h->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    }
     if (hipStreamCreate(&h->stream) != hipSuccess)
         return fail(set_err(h, BRP_ERR_DEVICE, "hipStreamCreate failed"));
     h->ev.resize(8);
@@ -1837,6 +1900,48 @@ int brp_set_clock(brp_handle *h, int64_t nstep, double time, double carry) {
     h->t = time;
     h->tx = carry;
     h->first_step = nstep == 0 ? 1 : 0;
+    return BRP_OK;
+}
+
+int brp_get_bubbles_noalloc(brp_handle *h, int32_t which, double *dst) {
+    if (!h || !dst || which < 0 || which >= BRP_BUB_COUNT) return BRP_ERR_INVALID;
+    if (!h->prepared) return set_err(h, BRP_ERR_STATE, "get_bubbles_noalloc: call brp_prepare first");
+    const int n = h->nb;
+    if (n == 0) return BRP_OK;
+    if (n > kScratchBubbles)
+        return set_err(h, BRP_ERR_INVALID, "get_bubbles_noalloc: %d bubbles exceed its staging "
+                       "array of %d", n, kScratchBubbles);
+    if (!h->scratch_d) return set_err(h, BRP_ERR_DEVICE, "%s", h->scratch_err.c_str());
+    const int c = h->rp_cur;
+    const double *src = nullptr;
+    const int *isrc = nullptr;
+    int wi = -1;
+    switch (which) {
+        case BRP_BUB_X: src = h->b.x; break;
+        case BRP_BUB_Y: src = h->b.y; break;
+        case BRP_BUB_R_T: src = h->b.R_t; break;
+        case BRP_BUB_R_P: src = h->b.R[c]; break;
+        case BRP_BUB_R_N: src = h->b.Rn[c]; break;
+        case BRP_BUB_D1R: src = h->b.d1R[c]; break;
+        case BRP_BUB_PG: src = h->b.PG[c]; break;
+        case BRP_BUB_PL_P: src = h->b.PL[h->pl_p]; break;
+        case BRP_BUB_PL_N: src = h->b.PL[h->pl_n]; break;
+        case BRP_BUB_PL_M: src = h->b.PL[h->pl_m]; break;
+        case BRP_BUB_Q_B: src = h->b.Q_B; break;
+        case BRP_BUB_N_B: src = h->b.n_B; break;
+        case BRP_BUB_DT: src = h->b.bdt[c]; break;
+        case BRP_BUB_RE: src = h->b.re[c]; break;
+        case BRP_BUB_VX: src = h->b.vx; break;
+        case BRP_BUB_VY: src = h->b.vy; break;
+        case BRP_BUB_R0: src = h->b.R0; break;
+        case BRP_BUB_SUB: isrc = h->b.sub; break;
+        default: wi = which - BRP_BUB_IBM_X;
+    }
+    hipLaunchKernelGGL(k_bubbles_to_caller, dim3((n + 255) / 256), dim3(256), 0, h->stream, n,
+                       h->perm_d, src, isrc, h->b.x, h->b.y, h->g.rdx, h->g.rdy, wi, h->scratch_d);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(dst, h->scratch_d, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return BRP_OK;
 }
 

@@ -245,6 +245,17 @@ int brp_set_field(brp_handle *h, int32_t field, const double *src, int64_t ld);
  * brp_set_bubbles) into dst. */
 int brp_get_bubbles(brp_handle *h, int32_t which, double *dst);
 
+/* brp_get_bubbles without any allocation: no hipMalloc, hipFree or other
+ * allocating call, so code that must not allocate (under HIP graph capture,
+ * in a tight serving loop) can read the bubbles.  No counterpart in the
+ * reference; it exists for those callers.  The handle must be prepared
+ * (brp_prepare or a step), else BRP_ERR_STATE.  The values are staged in one
+ * statically allocated device array per GPU of 4,194,304 doubles (32 MiB,
+ * held until the process ends): more bubbles than that give
+ * BRP_ERR_INVALID.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (from several host threads) race. */
+int brp_get_bubbles_noalloc(brp_handle *h, int32_t which, double *dst);
+
 /* Frozen schedule, for finite differences of the piecewise-smooth map that
  * brp_adjoint differentiates: mode 1 records every step's Picard count and,
  * for every RP launch, every bubble's accepted RP step sizes; mode 2 replays
