@@ -349,6 +349,26 @@ def adjoint_leg(brp, steps, window, every):
             "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())}
 
 
+def ascii_writer_leg(n=4096):
+    """The driver's ASCII field writer (cuda-rp_amd/bin/rawdata_bench): one
+    n x n field of save_ascii text, MB/s with the host's threads and with
+    one (the tool also checks the bytes against the serial ostream text)."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "cuda-rp_amd", "bin", "rawdata_bench")
+    threads = oracle_threads()
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([exe, d, str(n), str(threads)], capture_output=True, text=True,
+                           timeout=600)
+    out = {"grid": n, "ok": r.returncode == 0 and r.stdout.strip().endswith("OK")}
+    for line in r.stdout.splitlines():
+        if line.startswith("threads"):
+            t = int(line.split()[1].rstrip(":"))
+            out[f"MB_per_s_{t}_threads"] = float(line.split("=")[1].split()[0])
+            out["MB_per_field"] = float(line.split(":")[1].split()[0])
+    return out
+
+
 def calibrate_copy(mib=1024, reps=5):
     """A plain device copy of known size (reads and writes `mib` MiB per
     launch), run under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale
@@ -473,6 +493,8 @@ def main():
     sim_m.close()
 
     if not args.no_legs:
+        log("ASCII writer leg")
+        out["ascii_writer"] = ascii_writer_leg()
         log("2048^2 leg")
         out["focused_2048"] = focused_2048_leg(brp, W, args.leg_steps)
         log("thermal leg")

@@ -1327,7 +1327,9 @@ int brp_create(const brp_config *cfg, brp_handle **out) {
     h->fsz = h->prec == BRP_PREC_FP64 ? 8 : 4;
     h->N = c.grid_x;
     h->M = c.grid_y;
-    h->pitch = ((h->N + 66 + 63) / 64) * 64;
+    // columns -1..N+1 from offset kCol0 - 1, plus room for the grid kernels'
+    // last 4-cell group of a row and its right neighbour (columns up to N + 4)
+    h->pitch = ((h->N + kCol0 + 5 + 63) / 64) * 64;
     h->rows = h->M + 3;
     h->frame = (size_t)h->pitch * h->rows;
     auto fail = [&](int rc) {

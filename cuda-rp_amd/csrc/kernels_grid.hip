@@ -49,8 +49,33 @@ __device__ inline F rho_m_at(const GridParams &g, const int64_t *fgn, Window w, 
     return (F)((1.0 - f) * g.rho_inf);
 }
 
+// Four consecutive cells of one frame row as one vector access (16 bytes in
+// FP32, 32 in FP64).  Frame rows start 256-byte aligned and a thread's first
+// column is a multiple of 4, so the accesses are aligned; the pitch leaves
+// room for the last group of a row (brp_create sizes it).
+template <typename F>
+struct Vec4 {
+    typedef F type __attribute__((ext_vector_type(4)));
+};
+template <typename F>
+__device__ inline typename Vec4<F>::type ld4(const F *p) {
+    return *(const typename Vec4<F>::type *)p;
+}
+template <typename F>
+__device__ inline void st4(F *p, typename Vec4<F>::type v) {
+    *(typename Vec4<F>::type *)p = v;
+}
+
+constexpr int kGW = 4;     // cells per thread along a row
+constexpr int kGRows = 4;  // consecutive rows per thread
+constexpr int kGBlockRows = kBY * kGRows;  // rows per block (wavefronts stacked in y)
+
 // VelocityKernel (bubble_CUDA_kernel.cuh:635-677): vx at nodes i in [0, N],
-// rows j in [1, M]; vy at cells i in [1, N], node rows j in [0, M].
+// rows j in [1, M]; vy at cells i in [1, N], node rows j in [0, M].  A
+// wavefront covers 256 consecutive cells of a row (4 per lane); each thread
+// walks kGRows consecutive rows, so the p0 row j + 1 it loads for the vy
+// gradient is its next row's p0; the p0 column i + 4 of the vx gradient comes
+// from the next lane.  Per cell the arithmetic is the scalar update's.
 template <typename F>
 __global__ __launch_bounds__(256) void k_velocity(GridParams g, F *__restrict__ vx,
                                                    F *__restrict__ vy,
@@ -60,32 +85,72 @@ __global__ __launch_bounds__(256) void k_velocity(GridParams g, F *__restrict__ 
                                                    const F *__restrict__ cxb,
                                                    const F *__restrict__ cya,
                                                    const F *__restrict__ cyb) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    if (i > g.N) return;
+    typedef typename Vec4<F>::type V;
+    const int lane = threadIdx.x;
+    const int i0 = (blockIdx.x * kBX + lane) * kGW;
+    const int jb = blockIdx.y * kGBlockRows + threadIdx.y * kGRows;
+    if (jb > g.M) return;  // wave-uniform
+    const bool live = i0 <= g.N;
     const F dt = (F)g.dt;
     const F rdx = (F)g.rdx;
     const F half = (F)0.5;
-    // dt / s1 where both densities are rho_inf (outside the void-fraction
-    // window, or on the first step): s1 = (rho_inf + rho_inf) / 2 = rho_inf
-    // exactly, so the quotient is the same number as in the window's path
     const F dt_rho = dt / (((F)g.rho_inf + (F)g.rho_inf) * half);
-    const bool col_w = !first && i + 1 >= w.i0 && i <= w.i1;  // i or i + 1 in the window
-    const int jend = min((int)(blockIdx.y + 1) * kRB - 1, g.M);
-    for (int j = blockIdx.y * kRB + threadIdx.y; j <= jend; j += kBY) {
-        const int64_t o = frame_off(i, j, g.pitch);
-        const F pc = p0[o];
-        const bool in_w = col_w && j + 1 >= w.j0 && j <= w.j1;  // a density not rho_inf
-        const F rc = in_w ? rho_m_at<F>(g, fgn, w, first, i, j) : (F)g.rho_inf;
-        if (j >= 1) {
-            const F s2 = (-pc + p0[o + 1]) * rdx;
-            const F q = in_w ? dt / ((rc + rho_m_at<F>(g, fgn, w, first, i + 1, j)) * half) : dt_rho;
-            vx[o] = (vx[o] * cxa[i + 1] - q * s2) / cxb[i + 1];
+    F ca[kGW], cb[kGW];
+#pragma unroll
+    for (int e = 0; e < kGW; e++) {
+        const int i = min(i0 + e, g.N);
+        ca[e] = cxa[i + 1];
+        cb[e] = cxb[i + 1];
+    }
+    const int jend = min(jb + kGRows - 1, g.M);
+    V pc = live ? ld4(p0 + frame_off(i0, jb, g.pitch)) : V{};
+    for (int j = jb; j <= jend; j++) {
+        const int64_t o = frame_off(i0, j, g.pitch);
+        const V pn = live ? ld4(p0 + o + g.pitch) : V{};  // row j + 1
+        F pr = __shfl_down(pc.x, 1, 64);                   // p0(i0 + 4, j)
+        if (lane == 63 && live) pr = p0[o + kGW];
+        if (live) {
+            const F pcs[kGW + 1] = {pc.x, pc.y, pc.z, pc.w, pr};
+            const F pns[kGW] = {pn.x, pn.y, pn.z, pn.w};
+            const bool row_w = !first && j + 1 >= w.j0 && j <= w.j1;
+            const F ya = cya[j + 1], yb = cyb[j + 1];
+            V ox = ld4(vx + o), oy = ld4(vy + o);
+            F nx[kGW], ny[kGW];
+            const F oxs[kGW] = {ox.x, ox.y, ox.z, ox.w};
+            const F oys[kGW] = {oy.x, oy.y, oy.z, oy.w};
+#pragma unroll
+            for (int e = 0; e < kGW; e++) {
+                const int i = i0 + e;
+                const bool in_w = row_w && i + 1 >= w.i0 && i <= w.i1;  // a density not rho_inf
+                const F rc = in_w ? rho_m_at<F>(g, fgn, w, first, i, j) : (F)g.rho_inf;
+                {
+                    const F s2 = (-pcs[e] + pcs[e + 1]) * rdx;
+                    const F q = in_w ? dt / ((rc + rho_m_at<F>(g, fgn, w, first, i + 1, j)) * half)
+                                     : dt_rho;
+                    nx[e] = (oxs[e] * ca[e] - q * s2) / cb[e];
+                }
+                {
+                    const F s2 = (-pcs[e] + pns[e]) * rdx;  // rdx, as the reference (kernel.cuh:671)
+                    const F q = in_w ? dt / ((rc + rho_m_at<F>(g, fgn, w, first, i, j + 1)) * half)
+                                     : dt_rho;
+                    ny[e] = (oys[e] * ya - q * s2) / yb;
+                }
+            }
+            // vx at nodes 0..N of rows 1..M; vy at cells 1..N of rows 0..M
+            if (j >= 1) {
+                if (i0 + kGW - 1 <= g.N)
+                    st4(vx + o, V{nx[0], nx[1], nx[2], nx[3]});
+                else
+                    for (int e = 0; e < kGW; e++)
+                        if (i0 + e <= g.N) vx[o + e] = nx[e];
+            }
+            if (i0 >= 1 && i0 + kGW - 1 <= g.N)
+                st4(vy + o, V{ny[0], ny[1], ny[2], ny[3]});
+            else
+                for (int e = 0; e < kGW; e++)
+                    if (i0 + e >= 1 && i0 + e <= g.N) vy[o + e] = ny[e];
         }
-        if (i >= 1) {
-            const F s2 = (-pc + p0[o + g.pitch]) * rdx;  // rdx, as the reference (kernel.cuh:671)
-            const F q = in_w ? dt / ((rc + rho_m_at<F>(g, fgn, w, first, i, j + 1)) * half) : dt_rho;
-            vy[o] = (vy[o] * cya[j + 1] - q * s2) / cyb[j + 1];
-        }
+        pc = pn;
     }
 }
 
@@ -182,7 +247,10 @@ __global__ __launch_bounds__(1024) void k_velocity_ghosts(GridParams g, F *vx, F
 // f_g / f_gn / f_gm rotation of store_variables (1250-1259):
 //   pred = 3 f_g - 3 f_gn + f_gm  (exact in int64)  -> written over f_gm,
 //   which becomes the step's f_g; the old f_g becomes f_gn.
-// The residual max |p0_old - p0_new| goes to sc->resid[0].
+// The residual max |p0_old - p0_new| goes to sc->resid[0].  Layout as
+// k_velocity: 4 cells per lane, kGRows consecutive rows per thread; the vy row
+// j - 1 of the divergence is the previous row's vy, the vx column i0 - 1 comes
+// from the previous lane.  Per cell the arithmetic is the scalar update's.
 template <typename F>
 __global__ __launch_bounds__(256) void k_pressure_first(
     GridParams g, const F *__restrict__ vx, const F *__restrict__ vy, const F *__restrict__ pnx,
@@ -191,47 +259,102 @@ __global__ __launch_bounds__(256) void k_pressure_first(
     int64_t *__restrict__ fg_mp, Window w, const F *__restrict__ ax, const F *__restrict__ bx,
     const F *__restrict__ ay, const F *__restrict__ by, const double *__restrict__ xu,
     const double *__restrict__ rxp, StepScalars *sc) {
-    const int i = blockIdx.x * kBX + threadIdx.x;  // i in [0, N+1]
+    typedef typename Vec4<F>::type V;
+    const int lane = threadIdx.x;
+    const int i0 = (blockIdx.x * kBX + lane) * kGW;  // cells i0 .. i0 + 3 of [0, N+1]
+    const int jb = blockIdx.y * kGBlockRows + threadIdx.y * kGRows;
     double rmax = 0.0;
     bool bad = false;
-    if (i <= g.N + 1) {
-        const int jend = min((int)(blockIdx.y + 1) * kRB - 1, g.M + 1);
+    if (jb <= g.M + 1) {  // wave-uniform
+        const bool live = i0 <= g.N + 1;
         const F s3 = (F)g.s3;
         const F rdx = (F)g.rdx, rdy = (F)g.rdy;
-        const bool col_in_w = i >= w.i0 && i <= w.i1;
-        for (int j = blockIdx.y * kRB + threadIdx.y; j <= jend; j += kBY) {
-            const int64_t o = frame_off(i, j, g.pitch);
-            int64_t fpred = 0, fold = 0;
-            if (col_in_w && j >= w.j0 && j <= w.j1) {
-                const int64_t c = fg_cur[o];
-                fpred = 3 * c - 3 * fg_n[o] + fg_mp[o];
-                fold = c;
-                fg_mp[o] = fpred;
+        F xa[kGW], xb[kGW];
+        double xum[kGW], xup[kGW], rx[kGW];
+#pragma unroll
+        for (int e = 0; e < kGW; e++) {
+            const int i = min(max(i0 + e, 1), g.N);
+            xa[e] = ax[i + 1];
+            xb[e] = bx[i + 1];
+            if (g.cylindrical) {
+                xum[e] = xu[i];
+                xup[e] = xu[i + 1];
+                rx[e] = rxp[i + 1];
             }
-            if (i < 1 || i > g.N || j < 1 || j > g.M) continue;
-            // outside the window fpred = fold = 0 and the source term is -0
-            F s5h = (F)(-0.0);
-            if (fpred != 0 || fold != 0) {
-                const double s5d =
-                    -fg_to_d(fpred - fold) / g.dt / (1.0 - 0.5 * fg_to_d(fpred + fold));
-                s5h = (F)(0.5 * s5d);
+        }
+        const bool col_w = i0 + kGW - 1 >= w.i0 && i0 <= w.i1;
+        const int jend = min(jb + kGRows - 1, g.M + 1);
+        V vyl = live ? ld4(vy + frame_off(i0, jb - 1, g.pitch)) : V{};  // vy row jb - 1
+        for (int j = jb; j <= jend; j++) {
+            const int64_t o = frame_off(i0, j, g.pitch);
+            const V vxc = live ? ld4(vx + o) : V{};
+            const V vyc = live ? ld4(vy + o) : V{};
+            F vxl = __shfl_up(vxc.w, 1, 64);  // vx(i0 - 1, j)
+            if (lane == 0 && live) vxl = vx[o - 1];
+            if (live) {
+                int64_t fpred[kGW] = {0, 0, 0, 0}, fold[kGW] = {0, 0, 0, 0};
+                if (col_w && j >= w.j0 && j <= w.j1) {
+#pragma unroll
+                    for (int e = 0; e < kGW; e++) {
+                        const int i = i0 + e;
+                        if (i >= w.i0 && i <= w.i1) {
+                            const int64_t c = fg_cur[o + e];
+                            fpred[e] = 3 * c - 3 * fg_n[o + e] + fg_mp[o + e];
+                            fold[e] = c;
+                            fg_mp[o + e] = fpred[e];
+                        }
+                    }
+                }
+                if (j >= 1 && j <= g.M) {
+                    const V qx = ld4(pnx + o), qy = ld4(pny + o), qo = ld4(p0 + o);
+                    const F pxs[kGW] = {qx.x, qx.y, qx.z, qx.w};
+                    const F pys[kGW] = {qy.x, qy.y, qy.z, qy.w};
+                    const F p0s[kGW] = {qo.x, qo.y, qo.z, qo.w};
+                    const F vxs[kGW + 1] = {vxl, vxc.x, vxc.y, vxc.z, vxc.w};
+                    const F vys[kGW] = {vyc.x, vyc.y, vyc.z, vyc.w};
+                    const F vyls[kGW] = {vyl.x, vyl.y, vyl.z, vyl.w};
+                    const F ya = ay[j + 1], yb = by[j + 1];
+                    F npx[kGW], npy[kGW], np0[kGW];
+#pragma unroll
+                    for (int e = 0; e < kGW; e++) {
+                        // outside the window fpred = fold = 0 and the source term is -0
+                        F s5h = (F)(-0.0);
+                        if (fpred[e] != 0 || fold[e] != 0) {
+                            const double s5d = -fg_to_d(fpred[e] - fold[e]) / g.dt /
+                                               (1.0 - 0.5 * fg_to_d(fpred[e] + fold[e]));
+                            s5h = (F)(0.5 * s5d);
+                        }
+                        F s1x;
+                        if (g.cylindrical)
+                            s1x = (F)((-xum[e] * (double)vxs[e] + xup[e] * (double)vxs[e + 1]) *
+                                      g.rdx * rx[e]);
+                        else
+                            s1x = (-vxs[e] + vxs[e + 1]) * rdx;
+                        const F s1y = (-vyls[e] + vys[e]) * rdy;
+                        npx[e] = pxs[e] * xa[e] - s3 * (s1x + s5h) / xb[e];
+                        npy[e] = pys[e] * ya - s3 * (s1y + s5h) / yb;
+                        np0[e] = npx[e] + npy[e];
+                        const int i = i0 + e;
+                        if (i >= 1 && i <= g.N) {
+                            rmax = fmax(rmax, fabs((double)p0s[e] - (double)np0[e]));
+                            bad |= !isfinite((double)np0[e]) || fabs(fg_to_d(fpred[e])) >= 1.0;
+                        }
+                    }
+                    if (i0 >= 1 && i0 + kGW - 1 <= g.N) {
+                        st4(px + o, V{npx[0], npx[1], npx[2], npx[3]});
+                        st4(py + o, V{npy[0], npy[1], npy[2], npy[3]});
+                        st4(p0 + o, V{np0[0], np0[1], np0[2], np0[3]});
+                    } else {
+                        for (int e = 0; e < kGW; e++)
+                            if (i0 + e >= 1 && i0 + e <= g.N) {
+                                px[o + e] = npx[e];
+                                py[o + e] = npy[e];
+                                p0[o + e] = np0[e];
+                            }
+                    }
+                }
             }
-            F s1x;
-            if (g.cylindrical)
-                s1x = (F)((-xu[i] * (double)vx[o - 1] + xu[i + 1] * (double)vx[o]) * g.rdx *
-                          rxp[i + 1]);
-            else
-                s1x = (-vx[o - 1] + vx[o]) * rdx;
-            const F s1y = (-vy[o - g.pitch] + vy[o]) * rdy;
-            const F ppx = pnx[o] * ax[i + 1] - s3 * (s1x + s5h) / bx[i + 1];
-            const F ppy = pny[o] * ay[j + 1] - s3 * (s1y + s5h) / by[j + 1];
-            px[o] = ppx;
-            py[o] = ppy;
-            const F pnew = ppx + ppy;
-            const double r = fabs((double)p0[o] - (double)pnew);
-            rmax = fmax(rmax, r);
-            bad |= !isfinite((double)pnew) || fabs(fg_to_d(fpred)) >= 1.0;
-            p0[o] = pnew;
+            vyl = vyc;
         }
     }
     if (bad) atomicOr(&sc->flags, 1);
@@ -365,8 +488,8 @@ template <typename F>
 void launch_velocity(hipStream_t s, const GridParams &g, F *vx, F *vy, const F *p0,
                      const int64_t *fgn, Window w, int first_step, const F *cxa, const F *cxb,
                      const F *cya, const F *cyb) {
-    dim3 blk(kBX, kBY);
-    dim3 grd((g.N + 1 + kBX - 1) / kBX, (g.M + 1 + kRB - 1) / kRB);
+    dim3 blk(kBX, kBY);  // columns 0..N, rows 0..M
+    dim3 grd((g.N + 1 + kBX * kGW - 1) / (kBX * kGW), (g.M + 1 + kGBlockRows - 1) / kGBlockRows);
     hipLaunchKernelGGL(k_velocity<F>, grd, blk, 0, s, g, vx, vy, p0, fgn, w, first_step, cxa,
                        cxb, cya, cyb);
 }
@@ -382,8 +505,8 @@ void launch_pressure_first(hipStream_t s, const GridParams &g, const F *vx, cons
                            const int64_t *fg_cur, const int64_t *fg_n, int64_t *fg_mp, Window w,
                            const F *ax, const F *bx, const F *ay, const F *by, const double *xu,
                            const double *rxp, StepScalars *sc) {
-    dim3 blk(kBX, kBY);
-    dim3 grd((g.N + 2 + kBX - 1) / kBX, (g.M + 2 + kRB - 1) / kRB);
+    dim3 blk(kBX, kBY);  // columns 0..N+1, rows 0..M+1
+    dim3 grd((g.N + 2 + kBX * kGW - 1) / (kBX * kGW), (g.M + 2 + kGBlockRows - 1) / kGBlockRows);
     hipLaunchKernelGGL(k_pressure_first<F>, grd, blk, 0, s, g, vx, vy, pnx, pny, px, py, p0,
                        fg_cur, fg_n, fg_mp, w, ax, bx, ay, by, xu, rxp, sc);
 }

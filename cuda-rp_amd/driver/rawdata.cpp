@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -127,29 +128,94 @@ void RawWriter::run() {
     }
 }
 
+// One `x<TAB>y<TAB>value` text line per point, formatted as the default
+// ostream insertion of a double: the %g conversion with six significant
+// digits, which std::to_chars(general, 6) produces without a locale or a
+// format string.
+static inline void put_g(std::string &out, double v) {
+    char b[32];
+    const std::to_chars_result r = std::to_chars(b, b + sizeof b, v, std::chars_format::general, 6);
+    out.append(b, (size_t)(r.ptr - b));
+}
+
+void format_ascii_rows(const Field &f, int i_lo, int X, double dx, double dy, int j_begin,
+                       int j_end, std::string &out, double *mn, double *mx) {
+    std::vector<std::string> xs((size_t)(X - i_lo + 1));
+    for (int i = i_lo; i <= X; i++) put_g(xs[(size_t)(i - i_lo)], (double)i * dx);
+    for (int j = j_begin; j < j_end; j++) {
+        std::string y;
+        put_g(y, (double)j * dy);
+        for (int i = i_lo; i <= X; i++) {
+            const double v = f.at(std::abs(i), j);
+            out += xs[(size_t)(i - i_lo)];
+            out += '\t';
+            out += y;
+            out += '\t';
+            put_g(out, v);
+            out += '\n';
+            *mx = std::max(*mx, v);
+            *mn = std::min(*mn, v);
+        }
+        out += '\n';
+    }
+}
+
+bool write_ascii_field(const std::string &path, const Field &f, int X, int Y, bool cylindrical,
+                       double dx, double dy, int threads, double *mn, double *mx) {
+    const int i_lo = cylindrical ? -X : 0;
+    const int rows = Y + 1;
+    constexpr int kBlockRows = 16;
+    const int nblk = (rows + kBlockRows - 1) / kBlockRows;
+    threads = std::max(1, std::min(threads, nblk));
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    // Rounds of `threads` consecutive row blocks: formatted in parallel into
+    // buffers that are reused from round to round (so their pages are
+    // touched once), then written in order.
+    std::vector<std::string> buf((size_t)threads);
+    std::vector<double> lo((size_t)threads, 1e300), hi((size_t)threads, -1e300);
+    bool ok = true;
+    for (int b0 = 0; b0 < nblk && ok; b0 += threads) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads && b0 + t < nblk; t++)
+            pool.emplace_back([&, t] {
+                const int j0 = (b0 + t) * kBlockRows, j1 = std::min(rows, j0 + kBlockRows);
+                buf[(size_t)t].clear();
+                format_ascii_rows(f, i_lo, X, dx, dy, j0, j1, buf[(size_t)t], &lo[(size_t)t],
+                                  &hi[(size_t)t]);
+            });
+        for (auto &th : pool) th.join();
+        for (int t = 0; t < threads && b0 + t < nblk; t++)
+            ok = ok && std::fwrite(buf[(size_t)t].data(), 1, buf[(size_t)t].size(), fp) ==
+                           buf[(size_t)t].size();
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    if (mn)
+        for (int t = 0; t < threads; t++) {
+            *mn = std::min(*mn, lo[(size_t)t]);
+            *mx = std::max(*mx, hi[(size_t)t]);
+        }
+    return ok;
+}
+
+static int writer_threads() {
+    const char *e = std::getenv("OMP_NUM_THREADS");
+    const int n = e ? std::atoi(e) : 0;
+    return n > 0 ? n : std::max(1, (int)std::thread::hardware_concurrency());
+}
+
 // save_ascii (src/output_styles.cpp:182-327): one `x<TAB>y<TAB>value` line per
 // point, j = 0..Y outer, i = (cylindrical ? -X : 0)..X inner with the value at
-// |i|, a blank line after each row; default ostream formatting.
+// |i|, a blank line after each row; default ostream formatting.  At 4096^2 one
+// field is ~0.5 GB of text, so rows are formatted on several threads.
 void RawWriter::write_ascii(const Snapshot &s) {
     const int X = geo_.X, Y = geo_.Y;
-    const int i_lo = geo_.cylindrical ? -X : 0;
+    const int nt = writer_threads();
     auto field_file = [&](const char *prefix, const Field &f, double *mn, double *mx) {
-        std::ofstream out(step_name(dir_, prefix, s.step, ".txt"));
-        if (!out) {
+        const std::string path = step_name(dir_, prefix, s.step, ".txt");
+        if (!write_ascii_field(path, f, X, Y, geo_.cylindrical, geo_.dx, geo_.dy, nt, mn, mx)) {
             std::lock_guard<std::mutex> lk(m_);
-            err_ = "cannot write " + step_name(dir_, prefix, s.step, ".txt");
-            return;
-        }
-        for (int j = 0; j <= Y; j++) {
-            for (int i = i_lo; i <= X; i++) {
-                const double v = f.at(std::abs(i), j);
-                out << (double)i * geo_.dx << "\t" << (double)j * geo_.dy << "\t" << v << "\n";
-                if (mn) {
-                    *mx = std::max(*mx, v);
-                    *mn = std::min(*mn, v);
-                }
-            }
-            out << "\n";
+            err_ = "cannot write " + path;
         }
     };
     if (s.has_fg) field_file("fg_step_", s.fg, &fg_min, &fg_max);
