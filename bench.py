@@ -319,10 +319,14 @@ def adjoint_leg(brp, steps, window, every):
     cfg, bub, src, box = cases.c4_case(precision=abi.PREC_MIXED)
     f = cases.make(brp.Simulation, cfg, bub, src)
     t0 = time.perf_counter()
-    done = 0
+    done, sub_total, pic_total, sub_max = 0, 0.0, 0, 0
+    nb = len(bub[0])
     while done < steps:
         m = min(500, steps - done)
-        f.step(m)
+        for st in f.step(m):
+            sub_total += st["rp_mean_substeps"] * nb * st["n_pic"]
+            pic_total += st["n_pic"]
+            sub_max = max(sub_max, st["rp_max_substeps"])
         done += m
         log(f"  adjoint workload forward step {done}: "
             f"{(time.perf_counter() - t0) / done * 1e3:.2f} ms/step")
@@ -345,8 +349,29 @@ def adjoint_leg(brp, steps, window, every):
             "adjoint_reverse_sweep_s": r["reverse_ms"] / 1e3,
             "memory_gb": r["memory_bytes"] / 1e9, "n_checkpoints": r["n_checkpoints"],
             "schedule_mismatch": r["schedule_mismatch"],
+            "forward_rp_substeps": sub_total, "forward_rp_launches": pic_total,
+            "forward_rp_max_substeps": sub_max,
             "grad_norm": float(np.linalg.norm(g)),
             "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())}
+
+
+def vf_accumulation(brp, sim):
+    """The void fraction of the simulation's current radii scattered with
+    each accumulation scheme of brp_scatter_variant, against FP64 atomics
+    (rounding of order 1e-16 per add; order-dependent, so a second FP64
+    scatter is compared too).  The library's own f_g is the int64 scheme."""
+    abi = brp.abi
+    ref = sim.scatter_variant(abi.VF_FP64_ATOMIC)
+    out = {"reference": "FP64 atomic scatter of the same radii"}
+    for name, mode in (("fp64_atomic_repeat", abi.VF_FP64_ATOMIC),
+                       ("int64_fixed_point", abi.VF_INT64),
+                       ("fp32_compensated", abi.VF_FP32_COMPENSATED),
+                       ("fp32_atomic", abi.VF_FP32_ATOMIC)):
+        v = sim.scatter_variant(mode)
+        out[name] = {"rel_l2": cases.rel_l2(v, ref), "rel_linf": cases.rel_linf(v, ref)}
+    out["int64_equals_fg"] = bool(np.array_equal(sim.scatter_variant(abi.VF_INT64),
+                                                 sim.field("fg")))
+    return out
 
 
 def ascii_writer_leg(n=4096):
@@ -403,6 +428,8 @@ def main():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-legs", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint leg (for profiling)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -415,6 +442,11 @@ def main():
     import importlib
     brp = importlib.import_module("cuda-rp_amd")
     abi = brp.abi
+    if args.adjoint_only:
+        log("adjoint leg")
+        print(json.dumps({"adjoint": adjoint_leg(brp, args.adjoint_steps, args.adjoint_window,
+                                                 args.adjoint_checkpoint)}), flush=True)
+        return
     K, W = args.steps, args.warmup
 
     cfg_m, bub = cases.c3_case(precision=abi.PREC_MIXED)
@@ -480,6 +512,8 @@ def main():
         log("window statistics, mixed vs FP64")
         mvd["window_after"] = window_statistics(sim_m, sim_d, cfg_m, args.stat_steps)
         out["mixed_vs_fp64"] = mvd
+        log("void-fraction accumulation schemes (FP64 run, end of its window)")
+        out["vf_accumulation"] = vf_accumulation(brp, sim_d)
         sim_d.close()
 
     if keep:

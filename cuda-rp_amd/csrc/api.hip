@@ -1040,16 +1040,21 @@ struct AdjState {
     double *Lx = nullptr, *Ly = nullptr;
     double *lRt = nullptr, *lPLp = nullptr, *lPLn = nullptr, *lPLm = nullptr;
     unsigned int *mismatch = nullptr;  // RP replays whose sub-step count differed
-    int *order = nullptr, *order_tmp = nullptr, *part = nullptr;  // longest-first RP order
-    int *sort_hist = nullptr;
+    // RP Jacobians of a run of steps (k_adj_rp_jac): columns, iteration
+    // descriptors, longest-first order and its bins and queue head
+    double *jac = nullptr;
+    JacSlot *slots = nullptr;
+    unsigned *ord = nullptr, *head = nullptr;
+    int *bins = nullptr;
 };
 
 // Reverse of one replayed step.  On entry `a` holds the adjoint of the state
 // after the step and the handle holds that state with `rec` its
-// intermediates; on exit `a` holds the adjoint of the state before it.
+// intermediates; jac[k] holds the RP Jacobian of Picard iteration k; on exit
+// `a` holds the adjoint of the state before it.
 template <typename F>
 void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, bool seed,
-                  double cJ, const int box[4], AdjState &a) {
+                  double cJ, const int box[4], const double *const *jac, AdjState &a) {
     hipStream_t s = h->stream;
     const GridParams &g = h->g;
     const bool bub = h->cfg.bub_enable && h->nb > 0;
@@ -1079,15 +1084,8 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
         launch_adj_pressure<F>(s, g, false, a.f, rec.fg_it[k], gC, k == n ? a.LfC : nullptr, a.Lf,
                                LgC, nullptr, nullptr, ax, bx, ay, by);
         launch_adj_scatter(s, g, h->bp, bv, brec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
-        BubbleArrays bo = bv;  // work order from the forward's sub-step counts
-        bo.sub = brec.sub_it[k];
-        bo.order = a.order;
-        bo.order_tmp = a.order_tmp;
-        bo.sort_hist = a.sort_hist;
-        bo.part = a.part;
-        launch_order_partition(s, bo, h->bp.heavy_threshold, h->bp.extreme_threshold);
-        launch_adj_rp(s, g, h->bp, bv, 0, 0, 1, brec.PLp_it[k], a.lRt, k == n ? a.LR : none,
-                      a.LRs, a.lPLp, a.lPLn, a.lPLm, brec.sub_it[k], a.mismatch, a.order);
+        launch_adj_rp_apply(s, bv.n, jac[k], a.lRt, k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn,
+                            a.lPLm);
         launch_adj_gather<F>(s, g, h->bp, bv, (const F *)rec.p0_it[k - 1], a.lPLp,
                              k == n ? a.LA : nullptr, a.f.p0, a.Lx, a.Ly);
     }
@@ -1204,10 +1202,42 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
                        &a.LRs[3], &a.LA, &a.LB, &a.Lx, &a.Ly, &a.lRt, &a.lPLp, &a.lPLn, &a.lPLm};
     for (double **q : bubs)
         if ((rc = mem.get(h, q, nb))) return rc;
-    if ((rc = mem.get(h, &a.mismatch, 1)) || (rc = mem.get(h, &a.order, nb)) ||
-        (rc = mem.get(h, &a.order_tmp, nb)) || (rc = mem.get(h, &a.part, 2 * (nb / 256 + 2))) ||
-        (rc = mem.get(h, &a.sort_hist, 2048)))
-        return rc;
+    if ((rc = mem.get(h, &a.mismatch, 1))) return rc;
+    // RP Jacobians: 7 columns per bubble and Picard iteration, 35 in a
+    // step's last; computed for runs of steps that fit jac_budget bytes
+    // (at least one step) and at most 2^31 items per launch
+    auto jac_doubles = [&](int np) -> size_t {
+        return np > 0 ? nb * (size_t)(kJacCols * (np - 1) + kJacColsLast) : 0;
+    };
+    const size_t jac_budget = size_t(24) << 30;
+    size_t jac_cap = 0, item_cap = 0, slot_cap = 0;
+    if (bub) {
+        for (int sg = 0; sg < nseg; sg++) {  // the largest run of steps any segment needs
+            size_t dsum = 0, isum = 0, ssum = 0;
+            for (int st = std::min(n, (sg + 1) * K); st >= sg * K + 1; st--) {
+                const size_t dj = jac_doubles(npic[st - 1]);
+                const size_t ij = nb * (size_t)npic[st - 1];
+                if (ssum && ((dsum + dj) * 8 > jac_budget || isum + ij >= (size_t(1) << 31))) {
+                    dsum = isum = ssum = 0;
+                }
+                dsum += dj;
+                isum += ij;
+                ssum += npic[st - 1];
+                jac_cap = std::max(jac_cap, dsum);
+                item_cap = std::max(item_cap, isum);
+                slot_cap = std::max(slot_cap, ssum);
+            }
+        }
+        if (item_cap >= (size_t(1) << 31))
+            return set_err(h, BRP_ERR_INVALID, "one step's RP Jacobians exceed 2^31 items");
+        if ((rc = mem.get(h, &a.jac, std::max<size_t>(jac_cap, 1))) ||
+            (rc = mem.get(h, &a.slots, std::max<size_t>(slot_cap, 1))) ||
+            (rc = mem.get(h, &a.ord, std::max<size_t>(item_cap, 1))) ||
+            (rc = mem.get(h, &a.head, 1)) || (rc = mem.get(h, &a.bins, 4096)))
+            return rc;
+    }
+    std::vector<JacSlot> slots_h;
+    std::vector<std::vector<const double *>> jk(K);
 
     // reverse sweep: per segment, the steps are recomputed from the segment's
     // checkpoint with their intermediates recorded, then reversed
@@ -1224,9 +1254,55 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
                 return set_err(h, BRP_ERR_STATE, "replay of step %d took %d Picard iterations, not %d",
                                st, recs[st - s0].n_pic, npic[st - 1]);
         }
-        for (int st = s1; st >= s0; st--)
-            adjoint_step<F>(h, recs[st - s0], use_tape ? tape[st - 1] : recs[st - s0],
-                            st > n - ob.window, cJ, box, a);
+        // reversed in runs [lo, hi] whose RP Jacobians are computed together
+        for (int hi = s1; hi >= s0;) {
+            int lo = hi;
+            size_t dsum = 0, isum = 0, ssum = 0;
+            for (; lo >= s0; lo--) {
+                const int np = npic[lo - 1];
+                const size_t dj = jac_doubles(np), ij = nb * (size_t)np;
+                if (lo < hi &&
+                    (dsum + dj > jac_cap || isum + ij > item_cap || ssum + np > slot_cap))
+                    break;
+                dsum += dj;
+                isum += ij;
+                ssum += np;
+            }
+            lo++;
+            slots_h.clear();
+            double *jp = a.jac;
+            for (int st = lo; st <= hi; st++) {
+                const StepRecord &br = use_tape ? tape[st - 1] : recs[st - s0];
+                const int np = npic[st - 1];
+                std::vector<const double *> &v = jk[st - s0];
+                v.assign(np + 1, nullptr);
+                for (int k = 1; k <= np; k++) {
+                    JacSlot sl{};
+                    for (int q = 0; q < 6; q++) sl.rp[q] = br.rp[q];
+                    sl.PLm = br.PLm;
+                    sl.PLn = br.PLn;
+                    sl.PLp = br.PLp_it[k];
+                    sl.sub = br.sub_it[k];
+                    sl.jac = jp;
+                    sl.last = k == np;
+                    v[k] = jp;
+                    if (jp) jp += nb * (size_t)(k == np ? kJacColsLast : kJacCols);
+                    slots_h.push_back(sl);
+                }
+            }
+            if (bub && !slots_h.empty()) {
+                HIPCHK(h, hipMemcpyAsync(a.slots, slots_h.data(), sizeof(JacSlot) * slots_h.size(),
+                                         hipMemcpyHostToDevice, s));
+                launch_adj_rp_jacobians(s, h->bp, h->b, h->g.dt, a.slots, (int)slots_h.size(), a.ord,
+                                        a.bins, a.head, a.mismatch, h->cus);
+            }
+            for (int st = hi; st >= lo; st--)
+                adjoint_step<F>(h, recs[st - s0], use_tape ? tape[st - 1] : recs[st - s0],
+                                st > n - ob.window, cJ, box, jk[st - s0].data(), a);
+            // slots_h is rewritten by the next run: its copy must have completed
+            HIPCHK(h, hipStreamSynchronize(s));
+            hi = lo - 1;
+        }
     }
     HIPCHK(h, hipStreamSynchronize(s));
     const auto t2 = clk::now();

@@ -262,11 +262,30 @@ template <typename F>
 void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp, int n,
                        const double *xs, const double *ys, const F *vx, const F *vy, double *Lvx,
                        double *Lvy, double *Lx, double *Ly);
-void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
-                   const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
-                   const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
-                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch,
-                   const int *order);
+// One Picard iteration of one step for the RP Jacobians of the reverse
+// sweep: the step's RP start state and pressure history, the iteration's
+// gathered pressure and the forward's sub-step counts (taped), and where its
+// Jacobian columns go (35 x nb doubles when last, else 7 x nb).
+struct JacSlot {
+    const double *rp[6];  // R, Rn, d1R, PG, dt, re at the step's start
+    const double *PLm, *PLn, *PLp;
+    const int *sub;
+    double *jac;
+    int last;  // the step's last Picard iteration
+};
+constexpr int kJacCols = 7, kJacColsLast = 35;
+// The Jacobians of n_slots iterations (slots in device memory), ordered
+// longest-first (ord: n_slots x nb indices, bins: 4096 ints, head: one
+// counter); replays whose sub-step count differs from the taped one are
+// counted in *mismatch.
+void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const BubbleArrays &b,
+                             double dt_mix, const JacSlot *slots, int n_slots, unsigned *ord,
+                             int *bins, unsigned *head, unsigned *mismatch, int cus);
+// The transpose of one iteration's RP map: LRe (the adjoint of the step's
+// final RP state) only for the last iteration, else null.
+void launch_adj_rp_apply(hipStream_t s, int nb, const double *jac, const double *lRt,
+                         double *const LRe[4], double *const LRs[4], double *lPLp, double *lPLn,
+                         double *lPLm);
 void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,
                            double *lPLn, double *lPLm);
 void launch_fill_d(hipStream_t s, double *p, size_t n, double v);

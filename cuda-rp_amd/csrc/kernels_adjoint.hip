@@ -12,8 +12,9 @@
 //                    including the derivative of the smooth-delta weights with
 //                    respect to the bubble position (bins are piecewise
 //                    constant and not differentiated)
-//   k_adj_rp         Rayleigh-Plesset over one mixture step at the frozen
-//                    sub-step schedule, seven tangent lanes per bubble
+//   k_adj_rp_jac / k_adj_rp_apply   Rayleigh-Plesset over one mixture step
+//                    at the frozen sub-step schedule: Jacobian columns from
+//                    seven tangent lanes per bubble, then their transpose
 // Grid adjoints are FP64 in both precision modes; scatter-type transposes
 // (gather adjoint, motion adjoint) use FP64 atomics, every other kernel is a
 // gather and writes each cell from one thread.
@@ -21,6 +22,8 @@
 #include "rp_fast.h"
 
 #include <math.h>
+
+#include <algorithm>
 
 namespace brp {
 namespace {
@@ -469,61 +472,185 @@ __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams b
     Ly[idx] = ly + dt * (lx * dvx_dy + ly * dvy_dy);
 }
 
-// Adjoint of one RP call (k_bubble_persistent's BubbleRadiusKernel part):
-// eight lanes per bubble, lane d < 7 runs the sub-step in (value, tangent)
-// arithmetic with the tangent seeded on input d (0..3 start state R, Rn,
-// d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m).  The value part repeats the forward's
-// operations exactly, so each lane takes the forward's sub-steps; lane 7
-// only counts them for the schedule check.  Each lane ends with
-//   lam_d = sum_o LRe_o dOut_o/dIn_d + lRt dR_t/dIn_d.
-// LRe (the adjoint of the step's final RP state) is null except for the last
-// Picard iteration.  Outputs: LRs[d] += (d < 4), lPLp = (d = 4),
-// lPLn += (d = 5), lPLm += (d = 6); sub (if not null) holds the forward's
-// sub-step counts, and a lane-7 mismatch is counted in *mismatch.  order
-// (if not null) lists the bubbles longest-first, so the long integrations
-// start in the first workgroups.
-__global__ __launch_bounds__(256) void k_adj_rp(GridParams g, BubbleParams bp, BubbleArrays b,
-                                                 int cur, int plm, int pln, const double *PLp,
-                                                 const double *lRt, const double *LRe0,
-                                                 const double *LRe1, const double *LRe2,
-                                                 const double *LRe3, double *LRs0, double *LRs1,
-                                                 double *LRs2, double *LRs3, double *lPLp,
-                                                 double *lPLn, double *lPLm, const int *sub,
-                                                 unsigned int *mismatch, const int *order) {
-    __shared__ double vtab[kRpVSize];  // the U and Y tables
-    rp_vtab_stage(bp.rp_shared, vtab);
+// Adjoint of the RP calls (k_bubble_persistent's BubbleRadiusKernel part).
+// The RP map of one Picard iteration, (R, Rn, d1R, PG, PL_p, PL_n, PL_m) ->
+// (R', Rn', d1R', PG', R_t), depends only on the forward's taped inputs, so
+// its Jacobian is computed for every Picard iteration of a run of steps in
+// one launch (k_adj_rp_jac, after the items are ordered longest-first by
+// k_jac_rank), and each reverse step applies its transpose to the step's
+// adjoints (k_adj_rp_apply).  Without it every iteration's integration would
+// be a launch of its own, bounded by that iteration's longest chain.
+
+// Bins of the longest-first order: sub-step counts 0..kJacBins-1 (larger
+// counts share the first bin), descending.
+constexpr int kJacBins = 4096;
+constexpr int kJacTile = 4096;  // items per workgroup of k_jac_rank
+
+__device__ inline int jac_bin(int sub) { return kJacBins - 1 - min(max(sub, 0), kJacBins - 1); }
+
+// Two passes over tiles of kJacTile items.  PLACE = false: global counts per
+// bin.  PLACE = true (after k_jac_scan turned the counts into offsets): each
+// tile reserves its range of every bin with one atomic and writes its items'
+// indices there.  The order inside a bin depends on timing; every item's
+// result does not.
+template <bool PLACE>
+__global__ __launch_bounds__(256) void k_jac_rank(const JacSlot *slots, int nb,
+                                                  unsigned n_items, int *bins,
+                                                  unsigned *ord) {
+    __shared__ int cnt[kJacBins];
+    __shared__ int base[PLACE ? kJacBins : 1];
+    for (int i = threadIdx.x; i < kJacBins; i += 256) cnt[i] = 0;
     __syncthreads();
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((gid >> 3) >= b.n) return;
-    const int idx = order ? order[gid >> 3] : gid >> 3, d = gid & 7;
-    const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
+    const unsigned t0 = blockIdx.x * (unsigned)kJacTile;
+    constexpr int kPer = kJacTile / 256;
+    int bin[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const unsigned q = t0 + r * 256 + threadIdx.x;
+        bin[r] = -1;
+        if (q < n_items) {
+            const unsigned sl = q / (unsigned)nb, idx = q - sl * (unsigned)nb;
+            bin[r] = jac_bin(slots[sl].sub[idx]);
+            if (!PLACE) atomicAdd(&cnt[bin[r]], 1);
+        }
+    }
+    if constexpr (!PLACE) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < kJacBins; i += 256)
+            if (cnt[i]) atomicAdd(&bins[i], cnt[i]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < kPer; r++)
+            if (bin[r] >= 0) atomicAdd(&cnt[bin[r]], 1);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kJacBins; i += 256) {
+            base[i] = cnt[i] ? atomicAdd(&bins[i], cnt[i]) : 0;
+            cnt[i] = 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kPer; r++)
+            if (bin[r] >= 0)
+                ord[base[bin[r]] + atomicAdd(&cnt[bin[r]], 1)] = t0 + r * 256 + threadIdx.x;
+    }
+}
+
+// Exclusive prefix sum of the kJacBins counts, in place (one workgroup).
+__global__ __launch_bounds__(1024) void k_jac_scan(int *bins) {
+    constexpr int kPer = kJacBins / 1024;
+    __shared__ int part[1024];
+    int v[kPer], sum = 0;
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        v[r] = bins[threadIdx.x * kPer + r];
+        sum += v[r];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        bins[threadIdx.x * kPer + r] = run;
+        run += v[r];
+    }
+}
+
+// One item of k_adj_rp_jac on lane d of its eight.
+__device__ inline void jac_item(const BubbleParams &bp, const BubbleArrays &b, double dt_mix,
+                                const JacSlot *slots, unsigned item, int d,
+                                const RPFastShared &fs, unsigned *mismatch) {
+    const unsigned nb = (unsigned)b.n;
+    const unsigned sl = item / nb, idx = item - sl * nb;
+    const JacSlot &S = slots[sl];
     const double R0 = b.R0[idx];
     const double PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
     const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
                          bp.T0 / (PG0 * R03)};
     auto seed = [&](double v, int k) { return dd(v, d == k ? 1.0 : 0.0); };
     RPLaneF<dd> L;
-    rp_begin_g<dd>(L, seed(b.R[cur][idx], 0), seed(b.Rn[cur][idx], 1), seed(b.d1R[cur][idx], 2),
-                   seed(b.PG[cur][idx], 3), b.bdt[cur][idx], b.re[cur][idx],
-                   seed(b.PL[plm][idx], 6), seed(b.PL[pln][idx], 5), seed(PLp[idx], 4), g.dt,
-                   bp.PL0);
+    rp_begin_g<dd>(L, seed(S.rp[0][idx], 0), seed(S.rp[1][idx], 1), seed(S.rp[2][idx], 2),
+                   seed(S.rp[3][idx], 3), S.rp[4][idx], S.rp[5][idx], seed(S.PLm[idx], 6),
+                   seed(S.PLn[idx], 5), seed(S.PLp[idx], 4), dt_mix, bp.PL0);
     while (L.remain > 0.0 && L.counter < bp.max_substeps) rp_substep_g(L, fk, fs);
-    double lam = lRt[idx] * L.Rt.d;
-    if (LRe0)
-        lam += LRe0[idx] * L.Rp.d + LRe1[idx] * L.Rn.d + LRe2[idx] * L.d1R.d +
-               LRe3[idx] * L.PG.d;
-    switch (d) {
-        case 0: LRs0[idx] += lam; break;
-        case 1: LRs1[idx] += lam; break;
-        case 2: LRs2[idx] += lam; break;
-        case 3: LRs3[idx] += lam; break;
-        case 4: lPLp[idx] = lam; break;
-        case 5: lPLn[idx] += lam; break;
-        case 6: lPLm[idx] += lam; break;
-        default:
-            if (sub && sub[idx] != L.counter) atomicAdd(mismatch, 1u);
-            break;
+    if (d < 7) {
+        double *J = S.jac + idx;
+        J[(size_t)d * nb] = L.Rt.d;
+        if (S.last) {
+            J[(size_t)(7 + d) * nb] = L.Rp.d;
+            J[(size_t)(14 + d) * nb] = L.Rn.d;
+            J[(size_t)(21 + d) * nb] = L.d1R.d;
+            J[(size_t)(28 + d) * nb] = L.PG.d;
+        }
+    } else if (S.sub[idx] != L.counter) {
+        atomicAdd(mismatch, 1u);
     }
+}
+
+// Jacobian columns of the RP map, eight lanes per item (one Picard
+// iteration of one bubble): lane d < 7 runs the sub-step in (value, tangent)
+// arithmetic with the tangent seeded on input d (0..3 start state R, Rn,
+// d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m).  The value part repeats the forward's
+// operations exactly, so each lane takes the forward's sub-steps; lane 7
+// only counts them and counts a difference from the taped count in
+// *mismatch.  Output: jac[c * nb + idx] for c = 7 o + d, o = 0 (R_t) and, in
+// the step's last iteration, o = 1..4 (R', Rn', d1R', PG').  Persistent:
+// each wavefront takes the next eight items of ord until none is left.
+__global__ __launch_bounds__(256) void k_adj_rp_jac(BubbleParams bp, BubbleArrays b, double dt_mix,
+                                                     const JacSlot *slots, const unsigned *ord,
+                                                     unsigned n_items, unsigned *head,
+                                                     unsigned *mismatch) {
+    __shared__ double vtab[kRpVSize];  // the U and Y tables
+    rp_vtab_stage(bp.rp_shared, vtab);
+    __syncthreads();
+    const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
+    const int lane = threadIdx.x & 63, d = lane & 7;
+    for (;;) {
+        unsigned first = 0;
+        if (lane == 0) first = atomicAdd(head, 8u);
+        first = __shfl(first, 0);
+        if (first >= n_items) break;
+        const unsigned q = first + (lane >> 3);
+        if (q < n_items) jac_item(bp, b, dt_mix, slots, ord[q], d, fs, mismatch);
+    }
+}
+
+// Transpose of one Picard iteration's RP map applied to its adjoints:
+//   lam_d = lRt dR_t/dIn_d (+ sum_o LRe_o dOut_o/dIn_d in the last iteration)
+// then LRs[d] += lam_d (d < 4), lPLp = lam_4, lPLn += lam_5, lPLm += lam_6.
+__global__ __launch_bounds__(256) void k_adj_rp_apply(int nb, const double *jac, const double *lRt,
+                                                       const double *LRe0, const double *LRe1,
+                                                       const double *LRe2, const double *LRe3,
+                                                       double *LRs0, double *LRs1, double *LRs2,
+                                                       double *LRs3, double *lPLp, double *lPLn,
+                                                       double *lPLm) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    const double *J = jac + idx;
+    const size_t n = (size_t)nb;
+    const double lr = lRt[idx];
+    double lam[7];
+#pragma unroll
+    for (int d = 0; d < 7; d++) lam[d] = lr * J[d * n];
+    if (LRe0) {
+        const double e0 = LRe0[idx], e1 = LRe1[idx], e2 = LRe2[idx], e3 = LRe3[idx];
+#pragma unroll
+        for (int d = 0; d < 7; d++)
+            lam[d] += e0 * J[(7 + d) * n] + e1 * J[(14 + d) * n] + e2 * J[(21 + d) * n] +
+                      e3 * J[(28 + d) * n];
+    }
+    LRs0[idx] += lam[0];
+    LRs1[idx] += lam[1];
+    LRs2[idx] += lam[2];
+    LRs3[idx] += lam[3];
+    lPLp[idx] = lam[4];
+    lPLn[idx] += lam[5];
+    lPLm[idx] += lam[6];
 }
 
 // Liquid-pressure history adjoints across the step boundary:
@@ -547,6 +674,11 @@ __global__ void k_adj_pl_history(int n, int picard, double *LA, double *LB, doub
 __global__ void k_fill_d(double *p, size_t n, double v) {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = v;
+}
+
+__global__ void k_fill_i(int *p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
 }
 
 }  // namespace
@@ -634,16 +766,41 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
                        vx, vy, Lvx, Lvy, Lx, Ly);
 }
 
-void launch_adj_rp(hipStream_t s, const GridParams &g, const BubbleParams &bp,
-                   const BubbleArrays &b, int cur, int plm, int pln, const double *PLp,
-                   const double *lRt, double *const LRe[4], double *const LRs[4], double *lPLp,
-                   double *lPLn, double *lPLm, const int *sub, unsigned int *mismatch,
-                   const int *order) {
-    if (b.n <= 0) return;
-    const long threads = (long)b.n * 8;
-    hipLaunchKernelGGL(k_adj_rp, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g, bp,
-                       b, cur, plm, pln, PLp, lRt, LRe[0], LRe[1], LRe[2], LRe[3], LRs[0], LRs[1],
-                       LRs[2], LRs[3], lPLp, lPLn, lPLm, sub, mismatch, order);
+void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const BubbleArrays &b,
+                             double dt_mix, const JacSlot *slots, int n_slots, unsigned *ord,
+                             int *bins, unsigned *head, unsigned *mismatch, int cus) {
+    if (b.n <= 0 || n_slots <= 0) return;
+    const unsigned n_items = (unsigned)((long)n_slots * b.n);
+    const unsigned tiles = (n_items + kJacTile - 1) / kJacTile;
+    hipLaunchKernelGGL(k_fill_i, dim3((kJacBins + 255) / 256), dim3(256), 0, s, bins, kJacBins);
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(256), 0, s, (int *)head, 1);
+    hipLaunchKernelGGL(k_jac_rank<false>, dim3(tiles), dim3(256), 0, s, slots, b.n, n_items, bins,
+                       ord);
+    hipLaunchKernelGGL(k_jac_scan, dim3(1), dim3(1024), 0, s, bins);
+    hipLaunchKernelGGL(k_jac_rank<true>, dim3(tiles), dim3(256), 0, s, slots, b.n, n_items, bins,
+                       ord);
+    static int per_cu = 0;
+    if (!per_cu) {
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k_adj_rp_jac, 256, 0) !=
+                hipSuccess ||
+            nblk < 1)
+            nblk = 1;
+        per_cu = nblk;
+    }
+    const long waves = ((long)n_items + 7) / 8;
+    const long blocks = std::min<long>((long)per_cu * cus, (waves + 3) / 4);
+    hipLaunchKernelGGL(k_adj_rp_jac, dim3((unsigned)blocks), dim3(256), 0, s, bp, b, dt_mix, slots,
+                       ord, n_items, head, mismatch);
+}
+
+void launch_adj_rp_apply(hipStream_t s, int nb, const double *jac, const double *lRt,
+                         double *const LRe[4], double *const LRs[4], double *lPLp, double *lPLn,
+                         double *lPLm) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_adj_rp_apply, dim3((nb + 255) / 256), dim3(256), 0, s, nb, jac, lRt,
+                       LRe[0], LRe[1], LRe[2], LRe[3], LRs[0], LRs[1], LRs[2], LRs[3], lPLp, lPLn,
+                       lPLm);
 }
 
 void launch_adj_pl_history(hipStream_t s, int n, int picard, double *LA, double *LB,

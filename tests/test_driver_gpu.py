@@ -177,6 +177,42 @@ def _butterworth():
     return f
 
 
+def _pid_loop(c, steps, start, carry=None, filt=None):
+    """The driver's per-pass loop on the oracle: the filtered focal y of
+    every step and, after `start`, focal_PID's output re-aiming the array;
+    carry = (control, err_total, d_err) from a previous pass, filt the
+    Butterworth filter, whose state carries over between passes (the
+    reference keeps it in file-scope statics)."""
+    o = ol.OracleSim(c)
+    o.prepare()
+    info = o.info()
+    fp = (info["fp_x"], info["fp_y"])
+    filt = filt or _butterworth()
+    focal = [0.0]
+    if carry:
+        control = [carry[0]]
+        err_tot, d_err = list(carry[1]), list(carry[2])
+        o.set_focus(fp[0], carry[0], carry[0] / 0.5 / np.sqrt(3.0))
+    else:
+        control = [fp[1]]
+        err_tot, d_err = [0.0, 0.0], [0.0, 0.0]
+    K, Pc = 0.2 * 0.60, 1000.0
+    D, I = K * 0.125 * Pc, K / (0.5 * Pc)
+    for n in range(1, steps + 1):
+        st = o.step(1)[0]
+        focal.append(filt(st["focal_y"]))
+        if n > start:
+            act, prev = (0.0, focal[-1]), (0.0, focal[-2])
+            err = [fp[0] - act[0], fp[1] - act[1]]
+            err_tot = [err_tot[0] + err[0], err_tot[1] + err[1]]
+            d_err = [prev[0] - act[0], prev[1] - act[1]]
+            uy = fp[1] + (K * err[1] + D * d_err[1] + I * err_tot[1])
+            uy = min(max(uy, fp[1]), c.grid_ly)
+            control.append(uy)
+            o.set_focus(fp[0], uy, uy / 0.5 / np.sqrt(3.0))
+    return focal, control, (control[-1], err_tot, d_err), o, filt
+
+
 def test_driver_focal_tracking_and_pid(tmp_path):
     """transducer.pid = 1: the driver's focal-point series (filtered) and the
     controller's outputs follow the same loop run on the oracle
@@ -221,3 +257,64 @@ def test_driver_focal_tracking_and_pid(tmp_path):
     np.testing.assert_allclose(fa[:, 1], focal, rtol=5e-6, atol=1e-12)
     np.testing.assert_allclose(fc[:, 1], control, rtol=5e-6)
     assert len(set(np.round(fc[:, 1], 9))) > 1  # the controller moved the focus
+
+
+def test_driver_pid_iterate_passes(tmp_path):
+    """transducer.pid_iterate = 2: two passes into DIR_I/ and DIR_II/
+    (bubbles.cpp:221-246); the second starts from the first pass's
+    controller state (its last control point and error sums,
+    bubble_CUDA.cu:470-475), re-aims the array from its first step and
+    continues the first pass's Butterworth filter.  Both
+    passes' focal-point and controller files against the same loop run on
+    the oracle."""
+    conf = tmp_path / "conf.txt"
+    conf.write_text(CONF.replace("focal_dist = 3.4e-3\n", "focal_dist = 3.4e-3\npid = 1\n"
+                                 "pid_start_step = 10\npid_iterate = 2\n"))
+    out = tmp_path / "raw"
+    r = subprocess.run([BIN, "-a", "-d", str(out), str(conf)], capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d1, d2 = tmp_path / "raw_I", tmp_path / "raw_II"
+    assert d1.is_dir() and d2.is_dir() and not out.exists()
+    c = conf_as_config()
+    c.update(tr_pid=1, tr_pid_start_step=10, tr_pid_iterate=2)
+    f1, c1, carry, _, filt = _pid_loop(c, 60, 10)
+    f2, c2, _, o2, _ = _pid_loop(c, 60, 10, carry, filt)
+    for d, focal, control in ((d1, f1, c1), (d2, f2, c2)):
+        fa = np.loadtxt(d / "focalpoint_actual_T.txt")
+        fc = np.loadtxt(d / "focalpoint_controller_T.txt")
+        np.testing.assert_allclose(fa[:, 1], focal, rtol=5e-6, atol=1e-12)
+        np.testing.assert_allclose(fc[:, 1], control, rtol=5e-6)
+        assert (d / "p0_step_00060.txt").exists() and (d / "runtime.txt").exists()
+    assert np.loadtxt(d2 / "focalpoint_controller_T.txt")[0, 1] == pytest.approx(c1[-1], rel=5e-6)
+    p0, _, _ = read_tuples(d2 / "p0_step_00060.txt", 48, 48)
+    want = o2.field("p0")[:49, :49]
+    np.testing.assert_allclose(p0, want, rtol=5e-6, atol=1e-6 * np.abs(want).max())
+
+
+def test_driver_cylindrical_ascii_is_mirrored(tmp_path):
+    """transducer.cylindrical = 1: the ASCII files hold the axisymmetric
+    field mirrored about the axis (output_styles.cpp:196): columns
+    i = -X..X at x = i dx, the value of column |i|, against the oracle's
+    cylindrical run."""
+    conf = tmp_path / "conf.txt"
+    conf.write_text(CONF.replace("cylindrical = 0", "cylindrical = 1"))
+    out = tmp_path / "raw"
+    r = subprocess.run([BIN, "-a", "-d", str(out), str(conf)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    X = Y = 48
+    a = np.loadtxt(out / "p0_step_00060.txt")
+    assert a.shape == ((2 * X + 1) * (Y + 1), 3)
+    v = a[:, 2].reshape(Y + 1, 2 * X + 1)
+    xs = a[:, 0].reshape(Y + 1, 2 * X + 1)
+    np.testing.assert_allclose(xs[0], np.arange(-X, X + 1) * 1e-4, rtol=1e-5, atol=1e-12)
+    np.testing.assert_array_equal(v[:, :X], v[:, X + 1:][:, ::-1])  # value at |i|
+    c = conf_as_config()
+    c.update(tr_cylindrical=1)
+    o = ol.OracleSim(c)
+    o.prepare()
+    o.step(60)
+    want = o.field("p0")[:Y + 1, :X + 1]
+    assert np.abs(want).max() > 1e4
+    np.testing.assert_allclose(v[:, X:], want, rtol=5e-6, atol=1e-6 * np.abs(want).max())
