@@ -355,6 +355,61 @@ def adjoint_leg(brp, steps, window, every):
             "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())}
 
 
+def c5_leg(brp, steps, window):
+    """brp_adjoint on the adjoint workload at scale (4096^2, 2^20 bubbles,
+    256 elements, checkpoints every ceil(sqrt(steps)) steps) beside a plain
+    forward of the same steps.  Its bubble tape (192 B per bubble-step) does
+    not fit the library's 96 GB tape budget, so the reverse sweep integrates
+    RP again between checkpoints.  Device memory is the library's account of
+    what the adjoint holds (checkpoints, step records, RP Jacobians) plus
+    the handle."""
+    abi = brp.abi
+    cfg, bub, src, box = cases.c5_case(precision=abi.PREC_MIXED)
+    out = {"workload": "4096x4096 grid, 32-cell PML, 1048576 log-normal bubbles in an 8x8 mm box, "
+                       "256 elements amp~U(0.3,0.7) MPa phase~U(0,2pi), 1 MHz, focal length 51.2 mm, "
+                       "mixed precision",
+           "steps": steps, "window": window}
+    f = cases.make(brp.Simulation, cfg, bub, src)
+    out["handle_gb"] = f.info()["device_bytes"] / 1e9
+    t0 = time.perf_counter()
+    done, sub_total, sub_max = 0, 0.0, 0
+    while done < steps:
+        m = min(500, steps - done)
+        for st in f.step(m):
+            sub_total += st["rp_mean_substeps"] * len(bub[0]) * st["n_pic"]
+            sub_max = max(sub_max, st["rp_max_substeps"])
+        done += m
+        log(f"  at-scale forward step {done}: {(time.perf_counter() - t0) / done * 1e3:.2f} ms/step")
+    out["forward_s"] = time.perf_counter() - t0
+    out["forward_rp_substeps"] = sub_total
+    out["forward_rp_max_substeps"] = sub_max
+    f.close()
+    a = cases.make(brp.Simulation, cfg, bub, src)
+    log(f"  at-scale adjoint: {steps} steps, window {window}")
+    t0 = time.perf_counter()
+    try:
+        r = a.adjoint(box, steps, window, 0)
+    except brp.BrpError as e:
+        out["error"] = str(e)
+        a.close()
+        return out
+    out["adjoint_s"] = time.perf_counter() - t0
+    a.close()
+    g = np.concatenate([r["grad_amp"], r["grad_phase"]])
+    out.update({"adjoint_over_forward": out["adjoint_s"] / out["forward_s"],
+                "adjoint_forward_sweep_s": r["forward_ms"] / 1e3,
+                "adjoint_reverse_sweep_s": r["reverse_ms"] / 1e3,
+                "J": r["J"], "adjoint_memory_gb": r["memory_bytes"] / 1e9,
+                "device_memory_gb": r["memory_bytes"] / 1e9 + out["handle_gb"],
+                "n_checkpoints": r["n_checkpoints"], "checkpoint_every": r["segment"],
+                "schedule_mismatch": r["schedule_mismatch"],
+                "grad_norm": float(np.linalg.norm(g)),
+                "grad_fg0_abs_max": float(np.abs(r["grad_fg0"]).max())})
+    log(f"  at-scale adjoint: {out['adjoint_s']:.1f} s vs forward {out['forward_s']:.1f} s, "
+        f"{out['device_memory_gb']:.1f} GB")
+    return out
+
+
 def vf_accumulation(brp, sim):
     """The void fraction of the simulation's current radii scattered with
     each accumulation scheme of brp_scatter_variant, against FP64 atomics
@@ -430,6 +485,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint leg (for profiling)")
+    ap.add_argument("--at-scale", action="store_true",
+                    help="only the adjoint at scale (4096^2, 2^20 bubbles, 256 elements)")
+    ap.add_argument("--at-scale-steps", type=int, default=8000)
+    ap.add_argument("--at-scale-window", type=int, default=1000)
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -442,6 +501,11 @@ def main():
     import importlib
     brp = importlib.import_module("cuda-rp_amd")
     abi = brp.abi
+    if args.at_scale:
+        log("adjoint at scale")
+        print(json.dumps({"adjoint_at_scale": c5_leg(brp, args.at_scale_steps,
+                                                      args.at_scale_window)}), flush=True)
+        return
     if args.adjoint_only:
         log("adjoint leg")
         print(json.dumps({"adjoint": adjoint_leg(brp, args.adjoint_steps, args.adjoint_window,

@@ -115,16 +115,26 @@ struct Arena {
 };
 
 // Intermediates of one step, kept for its adjoint.  Grid part: the
-// start-of-step pressure and f_g, the velocities after the update, every
-// pressure evaluation's p0 and every void-fraction iterate (k = 0 is the
-// prediction).  Bubble part: the RP start state and pressure history, the
-// positions before and after the motion, and each Picard iteration's
-// gathered pressure, clamped radius and sub-step count.
+// start-of-step pressure (whole frame) and, on the rectangle `rect` only,
+// the start-of-step f_g, the velocities after the update, every pressure
+// evaluation's p0 and every void-fraction iterate (k = 0 is the
+// prediction).  rect is the Picard window (which holds every nonzero f_g
+// cell and every bubble stencil) grown by kRecMargin cells, joined with the
+// objective box: the adjoint reads f_g everywhere (zero outside the window)
+// but p0 and the velocities only at bubble stencils and in the box.  Bubble
+// part: the RP start state and pressure history, the positions before and
+// after the motion, and each Picard iteration's gathered pressure, clamped
+// radius and sub-step count.
+constexpr int kRecMargin = 2;
 struct StepRecord {
     bool want_grid = true, want_bub = true;
     Arena *arena = nullptr;  // bubble part from here when set
+    int box[4] = {0, -1, 0, -1};  // objective box (inclusive cells) the rectangle covers
     // grid part
-    void *p0_start = nullptr;
+    void *p0_start = nullptr;                    // whole frame
+    Window rect{0, -1, 0, -1};
+    size_t rcap = 0;                             // cells of every rectangle buffer
+    std::vector<void *> rect_owned;
     int64_t *fgC = nullptr;                      // f_g at the start (the step's f_gn)
     void *vx_end = nullptr, *vy_end = nullptr;   // velocities after the update
     std::vector<void *> p0_it;       // [k] after evaluation k (k = 0..n_pic)
@@ -647,39 +657,89 @@ struct AdjMem {
     }
 };
 
+// Cells of a record rectangle.
+size_t rect_cells(const Window &w) {
+    return w.i0 > w.i1 || w.j0 > w.j1 ? 0 : (size_t)(w.i1 - w.i0 + 1) * (size_t)(w.j1 - w.j0 + 1);
+}
+
+// Copies rectangle w of a frame array (elements of `elem` bytes) into a
+// dense buffer.
+int copy_rect(brp_handle *h, void *dst, const void *src, size_t elem, const Window &w) {
+    const size_t cols = (size_t)(w.i1 - w.i0 + 1), rows = (size_t)(w.j1 - w.j0 + 1);
+    if (!rect_cells(w)) return BRP_OK;
+    const char *s0 = (const char *)src + frame_off(w.i0, w.j0, h->pitch) * elem;
+    HIPCHK(h, hipMemcpy2DAsync(dst, cols * elem, s0, (size_t)h->pitch * elem, cols * elem, rows,
+                               hipMemcpyDeviceToDevice, h->stream));
+    return BRP_OK;
+}
+
+int rec_alloc(brp_handle *h, StepRecord *r, size_t bytes, bool bub, void **out) {
+    void *q = nullptr;
+    if (bub && r->arena) {
+        q = r->arena->get(bytes);
+        if (!q) return set_err(h, BRP_ERR_DEVICE, "bubble tape allocation failed");
+    } else {
+        if (hipMalloc(&q, std::max<size_t>(bytes, 256)) != hipSuccess)
+            return set_err(h, BRP_ERR_DEVICE, "step record allocation failed");
+        r->owned.push_back(q);
+    }
+    *out = q;
+    return BRP_OK;
+}
+
+int rect_alloc(brp_handle *h, StepRecord *r, size_t elem, void **out) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(r->rcap * elem, 256)) != hipSuccess)
+        return set_err(h, BRP_ERR_DEVICE, "step record allocation failed");
+    r->rect_owned.push_back(q);
+    *out = q;
+    return BRP_OK;
+}
+
+// Sets the record's rectangle from the current Picard window and the box,
+// (re)allocating the rectangle buffers when they are too small; called once
+// per step, after the window update.
+int rec_set_rect(brp_handle *h, StepRecord *r) {
+    Window w{0, -1, 0, -1};
+    if (h->W.i0 <= h->W.i1 && h->W.j0 <= h->W.j1)
+        w = Window{std::max(h->W.i0 - kRecMargin, -1), std::min(h->W.i1 + kRecMargin, h->N + 1),
+                   std::max(h->W.j0 - kRecMargin, -1), std::min(h->W.j1 + kRecMargin, h->M + 1)};
+    window_union(w, r->box[0], r->box[1], r->box[2], r->box[3]);
+    r->rect = w;
+    const size_t need = std::max<size_t>(rect_cells(w), 1);
+    if (need <= r->rcap && r->fgC) return BRP_OK;
+    for (void *q : r->rect_owned) (void)hipFree(q);
+    r->rect_owned.clear();
+    r->rcap = std::max(need, r->rcap);
+    int rc;
+    void *q[3];
+    const size_t el[3] = {sizeof(int64_t), h->fsz, h->fsz};
+    for (int j = 0; j < 3; j++)
+        if ((rc = rect_alloc(h, r, el[j], &q[j]))) return rc;
+    r->fgC = (int64_t *)q[0];
+    r->vx_end = q[1];
+    r->vy_end = q[2];
+    for (size_t k = 0; k < r->p0_it.size(); k++) {
+        void *a, *b;
+        if ((rc = rect_alloc(h, r, h->fsz, &a)) || (rc = rect_alloc(h, r, sizeof(int64_t), &b)))
+            return rc;
+        r->p0_it[k] = a;
+        r->fg_it[k] = (int64_t *)b;
+    }
+    return BRP_OK;
+}
+
 int rec_grow(brp_handle *h, StepRecord *r, int k) {
     // k evaluations recorded so far need p0_it/fg_it[0..k-1] and PLp_it/Rt_it[1..k-1]
-    const size_t fb = h->fsz * h->frame, gb = sizeof(int64_t) * h->frame;
+    const size_t fb = h->fsz * h->frame;
     const size_t nbb = sizeof(double) * std::max(h->nb, 1);
-    auto get = [&](size_t bytes, bool bub, void **out) -> int {
-        void *q = nullptr;
-        if (bub && r->arena) {
-            q = r->arena->get(bytes);
-            if (!q) return set_err(h, BRP_ERR_DEVICE, "bubble tape allocation failed");
-        } else {
-            if (hipMalloc(&q, std::max<size_t>(bytes, 256)) != hipSuccess)
-                return set_err(h, BRP_ERR_DEVICE, "step record allocation failed");
-            r->owned.push_back(q);
-        }
-        *out = q;
-        return BRP_OK;
-    };
     int rc;
     if (!r->started) {
-        if (r->want_grid) {
-            void *q[4];
-            const size_t sz[4] = {fb, gb, fb, fb};
-            for (int j = 0; j < 4; j++)
-                if ((rc = get(sz[j], false, &q[j]))) return rc;
-            r->p0_start = q[0];
-            r->fgC = (int64_t *)q[1];
-            r->vx_end = q[2];
-            r->vy_end = q[3];
-        }
+        if (r->want_grid && (rc = rec_alloc(h, r, fb, false, &r->p0_start))) return rc;
         if (r->want_bub) {
             void *q[12];
             for (int j = 0; j < 12; j++)
-                if ((rc = get(nbb, true, &q[j]))) return rc;
+                if ((rc = rec_alloc(h, r, nbb, true, &q[j]))) return rc;
             r->x_start = (double *)q[0];
             r->y_start = (double *)q[1];
             for (int j = 0; j < 6; j++) r->rp[j] = (double *)q[2 + j];
@@ -693,10 +753,12 @@ int rec_grow(brp_handle *h, StepRecord *r, int k) {
     while ((int)std::max(r->p0_it.size(), r->PLp_it.size()) < k) {
         void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
         const bool first = std::max(r->p0_it.size(), r->PLp_it.size()) == 0;
-        if (r->want_grid && ((rc = get(fb, false, &a)) || (rc = get(gb, false, &b)))) return rc;
+        if (r->want_grid && r->rcap &&
+            ((rc = rect_alloc(h, r, h->fsz, &a)) || (rc = rect_alloc(h, r, sizeof(int64_t), &b))))
+            return rc;
         if (r->want_bub && !first &&
-            ((rc = get(nbb, true, &c)) || (rc = get(nbb, true, &d)) ||
-             (rc = get(sizeof(int) * std::max(h->nb, 1), true, &e))))
+            ((rc = rec_alloc(h, r, nbb, true, &c)) || (rc = rec_alloc(h, r, nbb, true, &d)) ||
+             (rc = rec_alloc(h, r, sizeof(int) * std::max(h->nb, 1), true, &e))))
             return rc;
         r->p0_it.push_back(a);
         r->fg_it.push_back((int64_t *)b);
@@ -707,14 +769,31 @@ int rec_grow(brp_handle *h, StepRecord *r, int k) {
     return BRP_OK;
 }
 
+// Device bytes a record holds.
+size_t rec_bytes(const brp_handle *h, const StepRecord &r) {
+    size_t b = 0;
+    if (!r.started) return 0;
+    const size_t nbb = sizeof(double) * std::max(h->nb, 1);
+    if (r.want_grid)
+        b += h->fsz * h->frame + r.rcap * (sizeof(int64_t) + 2 * h->fsz) +
+             r.p0_it.size() * r.rcap * (h->fsz + sizeof(int64_t));
+    if (r.want_bub && !r.arena && !r.PLp_it.empty())
+        b += 12 * nbb + (r.PLp_it.size() - 1) * (2 * nbb + nbb / 2);
+    return b;
+}
+
 void rec_free(StepRecord &r) {
     for (void *q : r.owned) (void)hipFree(q);
+    for (void *q : r.rect_owned) (void)hipFree(q);
     const bool g = r.want_grid, b = r.want_bub;
     Arena *ar = r.arena;
+    int bx[4];
+    std::copy(r.box, r.box + 4, bx);
     r = StepRecord{};
     r.want_grid = g;
     r.want_bub = b;
     r.arena = ar;
+    std::copy(bx, bx + 4, r.box);
 }
 
 // ---- the coupled step ----------------------------------------------------
@@ -791,10 +870,8 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         if (rec) {
             int rc = rec_grow(h, rec, 1);
             if (rc) return rc;
-            if (rec->want_grid) {
+            if (rec->want_grid)
                 HIPCHK(h, hipMemcpyAsync(rec->p0_start, h->p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
-                HIPCHK(h, hipMemcpyAsync(rec->fgC, h->fgb[old_cur], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
-            }
             if (rec->want_bub && h->nb > 0) {
                 const size_t nbb = sizeof(double) * h->nb;
                 const int c = h->rp_cur;
@@ -821,10 +898,6 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                  h->rxp_d, h->sc_d);
         launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
         HIPCHK(h, hipEventRecord(h->ev[3], s));
-        if (rec && rec->want_grid) {
-            HIPCHK(h, hipMemcpyAsync(rec->p0_it[0], p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
-            HIPCHK(h, hipMemcpyAsync(rec->fg_it[0], h->fgb[old_m], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
-        }
         HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr + sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIPCHK(h, hipStreamSynchronize(s));
         double r0;
@@ -834,6 +907,14 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         if (bub) {
             window_union(h->W, h->sc_h->bbox[0], h->sc_h->bbox[1], h->sc_h->bbox[2], h->sc_h->bbox[3]);
             if (h->cfg.tr_cylindrical && h->W.i0 <= 1 && h->W.i0 <= h->W.i1) h->W.i0 = 0;  // axis fold target
+        }
+        if (rec && rec->want_grid) {  // the window is final for this step
+            int rc = rec_set_rect(h, rec);
+            if (rc) return rc;
+            if ((rc = copy_rect(h, rec->fgC, h->fgb[old_cur], sizeof(int64_t), rec->rect)) ||
+                (rc = copy_rect(h, rec->p0_it[0], p0, sizeof(F), rec->rect)) ||
+                (rc = copy_rect(h, rec->fg_it[0], h->fgb[old_m], sizeof(int64_t), rec->rect)))
+                return rc;
         }
         int status = BRP_OK;
         long long lost = 0;  // bubbles the RP kernel did not write back
@@ -879,8 +960,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                     }
                 }
                 if (h->cfg.tr_cylindrical) launch_cyl_fold(s, h->g, h->fgb[target]);
-                if (rec && rec->want_grid)
-                    HIPCHK(h, hipMemcpyAsync(rec->fg_it[loop], h->fgb[target], sizeof(int64_t) * h->frame, hipMemcpyDeviceToDevice, s));
+                if (rec && rec->want_grid) {
+                    int rc = copy_rect(h, rec->fg_it[loop], h->fgb[target], sizeof(int64_t), rec->rect);
+                    if (rc) return rc;
+                }
                 if (rec && rec->want_bub) {
                     HIPCHK(h, hipMemcpyAsync(rec->PLp_it[loop], h->b.PL[h->pl_p], sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
                     HIPCHK(h, hipMemcpyAsync(rec->Rt_it[loop], h->b.R_t, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
@@ -892,8 +975,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                           h->fgb[target], h->fgb[old_cur], h->fgb[prev], h->W, ax,
                                           bx, ay, by, h->xu_d, h->rxp_d, h->sc_d, loop);
                 launch_pressure_ghosts<F>(s, h->g, p0, h->el_amp_d, h->el_phase_d, h->t);
-                if (rec && rec->want_grid)
-                    HIPCHK(h, hipMemcpyAsync(rec->p0_it[loop], p0, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+                if (rec && rec->want_grid) {
+                    int rc = copy_rect(h, rec->p0_it[loop], p0, sizeof(F), rec->rect);
+                    if (rc) return rc;
+                }
                 HIPCHK(h, hipEventRecord(h->ev[6], s));
                 HIPCHK(h, hipMemcpyAsync(h->sc_h, h->sc_d, hdr, hipMemcpyDeviceToHost, s));
                 HIPCHK(h, hipMemcpyAsync(&h->sc_h->resid[loop], &h->sc_d->resid[loop], 8,
@@ -978,8 +1063,10 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
         if (rec) {
             rec->n_pic = loop;
             if (rec->want_grid) {
-                HIPCHK(h, hipMemcpyAsync(rec->vx_end, vx, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
-                HIPCHK(h, hipMemcpyAsync(rec->vy_end, vy, sizeof(F) * h->frame, hipMemcpyDeviceToDevice, s));
+                int rc;
+                if ((rc = copy_rect(h, rec->vx_end, vx, sizeof(F), rec->rect)) ||
+                    (rc = copy_rect(h, rec->vy_end, vy, sizeof(F), rec->rect)))
+                    return rc;
             }
             if (rec->want_bub && h->nb > 0) {
                 HIPCHK(h, hipMemcpyAsync(rec->x_end, h->b.x, sizeof(double) * h->nb, hipMemcpyDeviceToDevice, s));
@@ -1059,7 +1146,11 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
     const GridParams &g = h->g;
     const bool bub = h->cfg.bub_enable && h->nb > 0;
     const int n = rec.n_pic;
-    const int64_t *gC = rec.fgC;
+    const Window &w = rec.rect;
+    const int rw = w.i1 - w.i0 + 1, rh = w.j1 - w.j0 + 1;
+    auto fr = [&](const void *p) { return Rect<F>{(const F *)p, w.i0, w.j0, rw, rh}; };
+    auto gr = [&](const int64_t *p) { return Rect<int64_t>{p, w.i0, w.j0, rw, rh}; };
+    const Rect<int64_t> gC = gr(rec.fgC);
     const F *ax = (const F *)h->ax, *bx = (const F *)h->bx, *ay = (const F *)h->ay,
             *by = (const F *)h->by;
     // the bubble arrays as the step saw them: positions after the motion, RP
@@ -1075,26 +1166,26 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
     bv.re[0] = brec.rp[5];
     bv.PL[0] = brec.PLm;
     bv.PL[1] = brec.PLn;
-    if (seed) launch_adj_seed<F>(s, g, (const F *)rec.p0_it[n], a.f.p0, box, cJ);
+    if (seed) launch_adj_seed<F>(s, g, fr(rec.p0_it[n]), a.f.p0, box, cJ);
     // start-of-step f_g adjoints: f_gn' = f_g and f_gm' = f_gn of the start
     double *LgC = a.LfN, *LfNs = a.LfM, *LfMs = a.LfC;
     double *const none[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int k = n; k >= 1; k--) {
         launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
-        launch_adj_pressure<F>(s, g, false, a.f, rec.fg_it[k], gC, k == n ? a.LfC : nullptr, a.Lf,
+        launch_adj_pressure<F>(s, g, false, a.f, gr(rec.fg_it[k]), gC, k == n ? a.LfC : nullptr, a.Lf,
                                LgC, nullptr, nullptr, ax, bx, ay, by);
         launch_adj_scatter(s, g, h->bp, bv, brec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
         launch_adj_rp_apply(s, bv.n, jac[k], a.lRt, k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn,
                             a.lPLm);
-        launch_adj_gather<F>(s, g, h->bp, bv, (const F *)rec.p0_it[k - 1], a.lPLp,
+        launch_adj_gather<F>(s, g, h->bp, bv, fr(rec.p0_it[k - 1]), a.lPLp,
                              k == n ? a.LA : nullptr, a.f.p0, a.Lx, a.Ly);
     }
     launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
-    launch_adj_pressure<F>(s, g, true, a.f, rec.fg_it[0], gC, n == 0 ? a.LfC : nullptr, nullptr,
+    launch_adj_pressure<F>(s, g, true, a.f, gr(rec.fg_it[0]), gC, n == 0 ? a.LfC : nullptr, nullptr,
                            LgC, LfNs, LfMs, ax, bx, ay, by);
     if (bub)
-        launch_adj_motion<F>(s, g, h->bp, h->nb, brec.x_start, brec.y_start, (const F *)rec.vx_end,
-                             (const F *)rec.vy_end, a.f.vx, a.f.vy, a.Lx, a.Ly);
+        launch_adj_motion<F>(s, g, h->bp, h->nb, brec.x_start, brec.y_start, fr(rec.vx_end),
+                             fr(rec.vy_end), a.f.vx, a.f.vy, a.Lx, a.Ly);
     launch_adj_velocity_ghosts(s, g, a.f.vx, a.f.vy);
     launch_adj_velocity<F>(s, g, a.f.vx, a.f.vy, (const F *)rec.p0_start, gC, rec.first,
                            (const F *)h->cxa, (const F *)h->cxb, (const F *)h->cya,
@@ -1151,6 +1242,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         }
         for (auto &r : recs) r.want_bub = false;
     }
+    for (auto &r : recs) std::copy(box, box + 4, r.box);
     struct Cleanup {
         std::vector<Snapshot> *a;
         Snapshot *c;
@@ -1343,10 +1435,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         // step records: per evaluation p0 and f_g, per iteration PL_p, R_t and
         // sub-step counts, per step start/end fields and bubble state
         size_t recb = arena.bytes();
-        for (const StepRecord &r : recs)
-            if (r.started)
-                recb += r.p0_it.size() * (h->fsz + 8) * fr + (3 * h->fsz + 8) * fr +
-                        (r.want_bub ? r.p0_it.size() * 20 * nb + 12 * 8 * nb : 0);
+        for (const StepRecord &r : recs) recb += rec_bytes(h, r);
         stats->memory_bytes = (int64_t)((nseg + 1) * snap_bytes(h) + mem.bytes + recb);
         stats->n_checkpoints = nseg;
         stats->segment = K;

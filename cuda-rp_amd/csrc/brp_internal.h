@@ -232,35 +232,48 @@ void launch_focal_point(hipStream_t s, const GridParams &g, const double *dT, co
 struct AdjField {
     double *p0, *px, *py, *pnx, *pny, *vx, *vy;
 };
+// A recorded rectangle of a frame array: logical cells [i0, i0 + w) x
+// [j0, j0 + h), stored row by row without padding; cells outside read 0.
+template <typename T>
+struct Rect {
+    const T *p;
+    int i0, j0, w, h;
+    __host__ __device__ bool in(int i, int j) const {
+        return (unsigned)(i - i0) < (unsigned)w && (unsigned)(j - j0) < (unsigned)h;
+    }
+    __device__ T at(int i, int j) const {
+        return in(i, j) ? p[(int64_t)(j - j0) * w + (i - i0)] : T(0);
+    }
+};
 template <typename F>
 void launch_box_sumsq(hipStream_t s, const GridParams &g, const F *p0, const int box[4],
                       double *out);
 template <typename F>
-void launch_adj_seed(hipStream_t s, const GridParams &g, const F *p0, double *Lp0,
+void launch_adj_seed(hipStream_t s, const GridParams &g, Rect<F> p0, double *Lp0,
                      const int box[4], double c);
 void launch_adj_pressure_ghosts(hipStream_t s, const GridParams &g, double *L,
                                 const double *el_amp, const double *el_phase, double t,
                                 double *gA, double *gP);
 template <typename F>
 void launch_adj_pressure(hipStream_t s, const GridParams &g, bool first, const AdjField &a,
-                         const int64_t *fk, const int64_t *gC, const double *extra, double *Lf,
+                         Rect<int64_t> fk, Rect<int64_t> gC, const double *extra, double *Lf,
                          double *LgC, double *LfN, double *LfM, const F *ax, const F *bx,
                          const F *ay, const F *by);
 void launch_adj_velocity_ghosts(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy);
 template <typename F>
 void launch_adj_velocity(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy,
-                         const F *p0, const int64_t *fgn, int first, const F *cxa, const F *cxb,
+                         const F *p0, Rect<int64_t> fgn, int first, const F *cxa, const F *cxb,
                          const F *cya, const F *cyb, double *Lp0, double *LfC);
 void launch_adj_scatter(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                         const BubbleArrays &b, const double *Rt, const double *Lf, double *lRt,
                         double *Lx, double *Ly);
 template <typename F>
 void launch_adj_gather(hipStream_t s, const GridParams &g, const BubbleParams &bp,
-                       const BubbleArrays &b, const F *p0, const double *lPL,
+                       const BubbleArrays &b, Rect<F> p0, const double *lPL,
                        const double *lPL_extra, double *Lp0, double *Lx, double *Ly);
 template <typename F>
 void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp, int n,
-                       const double *xs, const double *ys, const F *vx, const F *vy, double *Lvx,
+                       const double *xs, const double *ys, Rect<F> vx, Rect<F> vy, double *Lvx,
                        double *Lvy, double *Lx, double *Ly);
 // One Picard iteration of one step for the RP Jacobians of the reverse
 // sweep: the step's RP start state and pressure history, the iteration's

@@ -77,13 +77,12 @@ __global__ __launch_bounds__(256) void k_box_sumsq(GridParams g, const F *p0, in
 
 // dJ/dp0 of the step's final pressure: 2 c p0 over the box.
 template <typename F>
-__global__ void k_adj_seed(GridParams g, const F *p0, double *Lp0, int i0, int i1, int j0,
-                           int j1, double c) {
+__global__ void k_adj_seed(GridParams g, Rect<F> p0, double *Lp0, int i0, int i1, int j0,
+                          int j1, double c) {
     const int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = j0 + (int)blockIdx.y;
     if (i > i1 || j > j1) return;
-    const int64_t o = frame_off(i, j, g.pitch);
-    Lp0[o] += 2.0 * c * (double)p0[o];
+    Lp0[frame_off(i, j, g.pitch)] += 2.0 * c * (double)p0.at(i, j);
 }
 
 // Adjoint of k_pressure_ghosts: part 2 undone first, then part 1 with each
@@ -157,8 +156,8 @@ template <typename F, bool FIRST>
 __global__ __launch_bounds__(256) void k_adj_pressure(
     GridParams g, const double *__restrict__ Lp0, const double *__restrict__ Lpx,
     const double *__restrict__ Lpy, double *__restrict__ Lpnx, double *__restrict__ Lpny,
-    double *__restrict__ Lvx, double *__restrict__ Lvy, const int64_t *__restrict__ fk,
-    const int64_t *__restrict__ gC, const double *extra, double *__restrict__ Lf,
+    double *__restrict__ Lvx, double *__restrict__ Lvy, Rect<int64_t> fk, Rect<int64_t> gC,
+    const double *extra, double *__restrict__ Lf,
     double *__restrict__ LgC, double *__restrict__ LfN, double *LfM,
     const F *__restrict__ ax, const F *__restrict__ bx, const F *__restrict__ ay,
     const F *__restrict__ by) {
@@ -200,7 +199,7 @@ __global__ __launch_bounds__(256) void k_adj_pressure(
         double lf = extra ? extra[o] : 0.0;
         if (in_c) {
             double df, dg;
-            s5_partials(fgd(fk[o]), fgd(gC[o]), g.dt, &df, &dg);
+            s5_partials(fgd(fk.at(i, j)), fgd(gC.at(i, j)), g.dt, &df, &dg);
             const double ds5 = 0.5 * (a_c + b_c);
             lf += ds5 * df;
             LgC[o] += ds5 * dg;
@@ -293,7 +292,7 @@ __global__ __launch_bounds__(1024) void k_adj_velocity_ghosts(GridParams g, doub
 template <typename F>
 __global__ __launch_bounds__(256) void k_adj_velocity_a(
     GridParams g, const double *__restrict__ Lvx, const double *__restrict__ Lvy,
-    const F *__restrict__ p0, const int64_t *__restrict__ fgn, int first,
+    const F *__restrict__ p0, Rect<int64_t> fgn, int first,
     const F *__restrict__ cxb, const F *__restrict__ cyb, double *__restrict__ Lp0,
     double *__restrict__ LfC) {
     const int i = (int)(blockIdx.x * kAX + threadIdx.x);
@@ -302,7 +301,7 @@ __global__ __launch_bounds__(256) void k_adj_velocity_a(
     if (i > N + 1 || j > M + 1) return;
     const double dt = (double)(F)g.dt, rdx = (double)(F)g.rdx, rho = g.rho_inf;
     auto rho_at = [&](int a, int b) {
-        return first ? rho : (1.0 - fgd(fgn[frame_off(a, b, P)])) * rho;
+        return first ? rho : (1.0 - fgd(fgn.at(a, b))) * rho;
     };
     auto pv = [&](int a, int b) { return (double)p0[frame_off(a, b, P)]; };
     double lp = 0.0, lr = 0.0;  // adjoints of p0(i, j) and rho(i, j)
@@ -391,7 +390,7 @@ __global__ __launch_bounds__(256) void k_adj_scatter(GridParams g, BubbleParams 
 // evaluation's pressure): FP64 atomics into Lp0, position adjoints.
 template <typename F>
 __global__ __launch_bounds__(256) void k_adj_gather(GridParams g, BubbleParams bp,
-                                                     BubbleArrays b, const F *__restrict__ p0,
+                                                     BubbleArrays b, Rect<F> p0,
                                                      const double *lPL, const double *lPL_extra,
                                                      double *Lp0, double *Lx, double *Ly) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,7 +411,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(GridParams g, BubbleParams b
                 const double dwy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
                 const int64_t o = frame_off(i, j, g.pitch);
                 if (lam != 0.0) atomicAdd(&Lp0[o], lam * wx * wy);
-                const double p = (double)p0[o];
+                const double p = (double)p0.at(i, j);
                 sx += p * dwx * wy;
                 sy += p * wx * dwy;
             }
@@ -428,8 +427,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(GridParams g, BubbleParams b
 template <typename F>
 __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams bp, int n,
                                                      const double *xs, const double *ys,
-                                                     const F *__restrict__ vx,
-                                                     const F *__restrict__ vy, double *Lvx,
+                                                     Rect<F> vx, Rect<F> vy, double *Lvx,
                                                      double *Lvy, double *Lx, double *Ly) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
@@ -447,7 +445,7 @@ __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams b
                 const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
                 const int64_t o = frame_off(i, j, g.pitch);
                 if (lx != 0.0) atomicAdd(&Lvx[o], dt * lx * Dx * Dy);
-                const double v = (double)vx[o];
+                const double v = (double)vx.at(i, j);
                 dvx_dx += v * dDx * Dy;
                 dvx_dy += v * Dx * dDy;
             }
@@ -462,7 +460,7 @@ __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams b
                 const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
                 const int64_t o = frame_off(i, j, g.pitch);
                 if (ly != 0.0) atomicAdd(&Lvy[o], dt * ly * Dx * Dy);
-                const double v = (double)vy[o];
+                const double v = (double)vy.at(i, j);
                 dvy_dx += v * dDx * Dy;
                 dvy_dy += v * Dx * dDy;
             }
@@ -693,7 +691,7 @@ void launch_box_sumsq(hipStream_t s, const GridParams &g, const F *p0, const int
 }
 
 template <typename F>
-void launch_adj_seed(hipStream_t s, const GridParams &g, const F *p0, double *Lp0,
+void launch_adj_seed(hipStream_t s, const GridParams &g, Rect<F> p0, double *Lp0,
                      const int box[4], double c) {
     const int ni = box[1] - box[0] + 1, nj = box[3] - box[2] + 1;
     hipLaunchKernelGGL(k_adj_seed<F>, dim3((ni + 63) / 64, nj), dim3(64), 0, s, g, p0, Lp0, box[0],
@@ -709,7 +707,7 @@ void launch_adj_pressure_ghosts(hipStream_t s, const GridParams &g, double *L,
 
 template <typename F>
 void launch_adj_pressure(hipStream_t s, const GridParams &g, bool first, const AdjField &a,
-                         const int64_t *fk, const int64_t *gC, const double *extra, double *Lf,
+                         Rect<int64_t> fk, Rect<int64_t> gC, const double *extra, double *Lf,
                          double *LgC, double *LfN, double *LfM, const F *ax, const F *bx,
                          const F *ay, const F *by) {
     const dim3 blk(kAX, kAY), grid((g.N + 3 + kAX - 1) / kAX, (g.M + 3 + kAY - 1) / kAY);
@@ -730,7 +728,7 @@ void launch_adj_velocity_ghosts(hipStream_t s, const GridParams &g, double *Lvx,
 
 template <typename F>
 void launch_adj_velocity(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy,
-                         const F *p0, const int64_t *fgn, int first, const F *cxa, const F *cxb,
+                         const F *p0, Rect<int64_t> fgn, int first, const F *cxa, const F *cxb,
                          const F *cya, const F *cyb, double *Lp0, double *LfC) {
     const dim3 blk(kAX, kAY);
     const dim3 ga((g.N + 2 + kAX - 1) / kAX, (g.M + 2 + kAY - 1) / kAY);
@@ -750,7 +748,7 @@ void launch_adj_scatter(hipStream_t s, const GridParams &g, const BubbleParams &
 
 template <typename F>
 void launch_adj_gather(hipStream_t s, const GridParams &g, const BubbleParams &bp,
-                       const BubbleArrays &b, const F *p0, const double *lPL,
+                       const BubbleArrays &b, Rect<F> p0, const double *lPL,
                        const double *lPL_extra, double *Lp0, double *Lx, double *Ly) {
     if (b.n <= 0) return;
     hipLaunchKernelGGL(k_adj_gather<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, p0,
@@ -759,7 +757,7 @@ void launch_adj_gather(hipStream_t s, const GridParams &g, const BubbleParams &b
 
 template <typename F>
 void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp, int n,
-                       const double *xs, const double *ys, const F *vx, const F *vy, double *Lvx,
+                       const double *xs, const double *ys, Rect<F> vx, Rect<F> vy, double *Lvx,
                        double *Lvy, double *Lx, double *Ly) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_adj_motion<F>, dim3((n + 255) / 256), dim3(256), 0, s, g, bp, n, xs, ys,
@@ -818,21 +816,21 @@ void launch_fill_d(hipStream_t s, double *p, size_t n, double v) {
 #define BRP_ADJ_INSTANTIATE(F)                                                                  \
     template void launch_box_sumsq<F>(hipStream_t, const GridParams &, const F *, const int[4], \
                                       double *);                                                \
-    template void launch_adj_seed<F>(hipStream_t, const GridParams &, const F *, double *,      \
+    template void launch_adj_seed<F>(hipStream_t, const GridParams &, Rect<F>, double *,        \
                                      const int[4], double);                                     \
     template void launch_adj_pressure<F>(hipStream_t, const GridParams &, bool, const AdjField &, \
-                                         const int64_t *, const int64_t *, const double *,      \
+                                         Rect<int64_t>, Rect<int64_t>, const double *,          \
                                          double *, double *, double *, double *, const F *,     \
                                          const F *, const F *, const F *);                      \
     template void launch_adj_velocity<F>(hipStream_t, const GridParams &, double *, double *,   \
-                                         const F *, const int64_t *, int, const F *, const F *, \
+                                         const F *, Rect<int64_t>, int, const F *, const F *,   \
                                          const F *, const F *, double *, double *);             \
     template void launch_adj_gather<F>(hipStream_t, const GridParams &, const BubbleParams &,   \
-                                       const BubbleArrays &, const F *, const double *,         \
+                                       const BubbleArrays &, Rect<F>, const double *,           \
                                        const double *, double *, double *, double *);           \
     template void launch_adj_motion<F>(hipStream_t, const GridParams &, const BubbleParams &,   \
-                                       int, const double *, const double *, const F *,          \
-                                       const F *, double *, double *, double *, double *);
+                                       int, const double *, const double *, Rect<F>, Rect<F>,   \
+                                       double *, double *, double *, double *);
 BRP_ADJ_INSTANTIATE(float)
 BRP_ADJ_INSTANTIATE(double)
 

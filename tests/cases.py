@@ -116,7 +116,7 @@ def focal_objective(sim, steps, box, window):
     return acc / (window * (i1 - i0 + 1) * (j1 - j0 + 1))
 
 
-# ---- the benchmark configurations (BASELINE configs C1-C5) -----------------
+# ---- the benchmark workloads C1-C5 ----------------------------------------
 FOCAL = 15e-3
 
 
@@ -198,3 +198,40 @@ def c4_case(precision=abi.PREC_MIXED, n=1024, nb=65536, n_elem=64, seed=4):
     dx = cfg.grid_lx / n
     ic, jc = int(round(fx / dx + 0.5)), int(round(fy / dx + 0.5))
     return cfg, (x, y, R0, None, fg_in), (amp, phase), (ic - 10, ic + 9, jc - 10, jc + 9)
+
+
+def _aperture_case(n, npml, focal, nb, box, n_elem, seed, precision):
+    """A grid of n^2 cells (dx = 0.1 mm) driven at 1 MHz by an n_elem-element
+    aperture (amplitudes ~ U(0.3, 0.7) MPa, phases ~ U(0, 2 pi)) focused at
+    `focal`, nb log-normal bubbles (median 3 um, sigma_log 0.4) uniform in a
+    box x box square at the focus (mean void fraction 1e-6 there), the
+    objective box 2 x 2 mm (20 x 20 cells) at the focus."""
+    cfg = abi.water_config(n, npml=npml, cfl=0.5, amp_pa=0.5e6, freq=1.0e6, focal_dist=focal,
+                           bubbles=True, fg0=1e-6, precision=precision)
+    r = rng(seed)
+    fx, fy = cfg.ext_focus_x, focal * 0.5 * np.sqrt(3.0)
+    x = fx + (r.random(nb) - 0.5) * box
+    y = fy + (r.random(nb) - 0.5) * box
+    R0 = np.exp(np.log(3e-6) + 0.4 * r.standard_normal(nb))
+    fg_in = 1e-6 * (box / 1e-4) ** 2 / nb
+    amp = 0.3e6 + 0.4e6 * r.random(n_elem)
+    phase = 2.0 * np.pi * r.random(n_elem)
+    dx = cfg.grid_lx / n
+    ic, jc = int(round(fx / dx + 0.5)), int(round(fy / dx + 0.5))
+    return cfg, (x, y, R0, None, fg_in), (amp, phase), (ic - 10, ic + 9, jc - 10, jc + 9)
+
+
+def c5_case(precision=abi.PREC_MIXED, seed=5):
+    """The adjoint at scale: 4096^2, 32-cell PML, 2^20 bubbles in an 8 x 8 mm
+    box at the focus, a 256-element aperture (512 amplitude and phase
+    parameters).  The focal length is 51.2 mm, so that the aperture spans 512
+    columns, two per element.  Returns cfg, bubbles, (amp, phase), box."""
+    return _aperture_case(4096, 32, 51.2e-3, 1 << 20, 8e-3, 256, seed, precision)
+
+
+def c5_replica_case(precision=abi.PREC_FP64, seed=5):
+    """The adjoint workload at scale reduced to 256^2 cells for the oracle:
+    the same medium, source model and parameter count (256 elements, one
+    column each: focal length 25.6 mm, 16-cell PML), 4096 bubbles in a
+    4 x 4 mm box at the focus, the 2 x 2 mm objective box."""
+    return _aperture_case(256, 16, 25.6e-3, 4096, 4e-3, 256, seed, precision)

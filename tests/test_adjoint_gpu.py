@@ -287,3 +287,59 @@ def test_c4_adjoint_against_frozen_differences(brp):
         print(f"{which} {k}: adjoint {got:.8e} FD {fd:.8e} rel err {err:.2e}", flush=True)
         assert err < 1e-3, (which, k, got, fd)
     print(f"C4 adjoint vs frozen-schedule differences: worst relative error {worst:.2e}")
+
+
+def test_c5_replica_against_oracle(brp):
+    """The adjoint workload at scale, reduced to 256^2 cells (256 elements,
+    4096 bubbles, FP64, 550 steps with J over the last 100, checkpoints every
+    ceil(sqrt(550)) steps): brp_adjoint against the oracle's frozen-schedule
+    central differences along two seeded directions -- every one of the 512
+    amplitudes and phases moved by +-1e-4 (relative, and radians), and every
+    cloud cell of the initial void-fraction map by +-1e-4 of its value --
+    agreement 1e-3.  A direction sums the responses of all its parameters,
+    so one pair of oracle runs checks the whole gradient against the
+    reference's equations."""
+    cfg, bub, (amp, phase), box = cases.c5_replica_case()
+    steps, window, h = 550, 100, 1e-4
+    g = cases.make(brp.Simulation, cfg, bub, (amp, phase))
+    fg_base = g.field("fg")
+    r = g.adjoint(box, steps, window, 0)
+    g.close()
+    assert r["schedule_mismatch"] == 0
+    rec = cases.make(ol.OracleSim, cfg, bub, (amp, phase))
+    rec.schedule_mode(1)
+    J0 = cases.focal_objective(rec, steps, box, window)
+    assert r["J"] == pytest.approx(J0, rel=1e-9)
+    q = lambda v: np.rint(v * 2.0 ** 60) * 2.0 ** -60
+    cloud = fg_base > 0.1 * fg_base.max()
+
+    def oracle_J(a, p, m=None):
+        o = cases.make(ol.OracleSim, cfg, bub, (a, p))
+        if m is not None:
+            o.set_field("fg", m)
+        o.schedule_from(rec)
+        o.schedule_mode(2)
+        J = cases.focal_objective(o, steps, box, window)
+        o.close()
+        return J
+
+    r6 = cases.rng(6)
+    sa = np.sign(r6.standard_normal(len(amp)))
+    sp = np.sign(r6.standard_normal(len(phase)))
+    fd = (oracle_J(amp * (1 + h * sa), phase + h * sp) -
+          oracle_J(amp * (1 - h * sa), phase - h * sp)) / 2
+    got = float(np.dot(r["grad_amp"], h * sa * amp) + np.dot(r["grad_phase"], h * sp))
+    err_src = abs(got - fd) / abs(fd)
+    print(f"source direction: adjoint {got:.8e} oracle FD {fd:.8e} rel err {err_src:.2e}",
+          flush=True)
+    d = np.sign(r6.standard_normal(fg_base.shape))
+    mp, mm = fg_base.copy(), fg_base.copy()
+    mp[cloud] = q(fg_base[cloud] * (1 + h * d[cloud]))
+    mm[cloud] = q(fg_base[cloud] * (1 - h * d[cloud]))
+    fd_f = (oracle_J(amp, phase, mp) - oracle_J(amp, phase, mm)) / 2
+    got_f = float(np.sum(r["grad_fg0"][cloud] * (mp - mm)[cloud] / 2))
+    err_fg = abs(got_f - fd_f) / abs(fd_f)
+    print(f"void-fraction direction: adjoint {got_f:.8e} oracle FD {fd_f:.8e} "
+          f"rel err {err_fg:.2e}", flush=True)
+    rec.close()
+    assert err_src < 1e-3 and err_fg < 1e-3
