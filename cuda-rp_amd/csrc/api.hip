@@ -133,6 +133,7 @@ struct StepRecord {
     // grid part
     void *p0_start = nullptr;                    // whole frame
     Window rect{0, -1, 0, -1};
+    Window win{0, -1, 0, -1};                    // the step's Picard window
     size_t rcap = 0;                             // cells of every rectangle buffer
     std::vector<void *> rect_owned;
     int64_t *fgC = nullptr;                      // f_g at the start (the step's f_gn)
@@ -706,6 +707,7 @@ int rec_set_rect(brp_handle *h, StepRecord *r) {
                    std::max(h->W.j0 - kRecMargin, -1), std::min(h->W.j1 + kRecMargin, h->M + 1)};
     window_union(w, r->box[0], r->box[1], r->box[2], r->box[3]);
     r->rect = w;
+    r->win = h->W;
     const size_t need = std::max<size_t>(rect_cells(w), 1);
     if (need <= r->rcap && r->fgC) return BRP_OK;
     for (void *q : r->rect_owned) (void)hipFree(q);
@@ -1151,6 +1153,7 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
     auto fr = [&](const void *p) { return Rect<F>{(const F *)p, w.i0, w.j0, rw, rh}; };
     auto gr = [&](const int64_t *p) { return Rect<int64_t>{p, w.i0, w.j0, rw, rh}; };
     const Rect<int64_t> gC = gr(rec.fgC);
+    const Window pw{rec.win.i0 - 2, rec.win.i1 + 2, rec.win.j0 - 2, rec.win.j1 + 2};
     const F *ax = (const F *)h->ax, *bx = (const F *)h->bx, *ay = (const F *)h->ay,
             *by = (const F *)h->by;
     // the bubble arrays as the step saw them: positions after the motion, RP
@@ -1172,8 +1175,11 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
     double *const none[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int k = n; k >= 1; k--) {
         launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
+        // before the step's last evaluation, the adjoint inputs are the gather
+        // stencils' (inside the Picard window): that evaluation's transpose is
+        // zero outside the window grown by the stencil's reach
         launch_adj_pressure<F>(s, g, false, a.f, gr(rec.fg_it[k]), gC, k == n ? a.LfC : nullptr, a.Lf,
-                               LgC, nullptr, nullptr, ax, bx, ay, by);
+                               LgC, nullptr, nullptr, ax, bx, ay, by, k == n ? nullptr : &pw);
         launch_adj_scatter(s, g, h->bp, bv, brec.Rt_it[k], a.Lf, a.lRt, a.Lx, a.Ly);
         launch_adj_rp_apply(s, bv.n, jac[k], a.lRt, k == n ? a.LR : none, a.LRs, a.lPLp, a.lPLn,
                             a.lPLm);
@@ -1182,7 +1188,7 @@ void adjoint_step(brp_handle *h, const StepRecord &rec, const StepRecord &brec, 
     }
     launch_adj_pressure_ghosts(s, g, a.f.p0, h->el_amp_d, h->el_phase_d, rec.t, a.gA, a.gP);
     launch_adj_pressure<F>(s, g, true, a.f, gr(rec.fg_it[0]), gC, n == 0 ? a.LfC : nullptr, nullptr,
-                           LgC, LfNs, LfMs, ax, bx, ay, by);
+                           LgC, LfNs, LfMs, ax, bx, ay, by, nullptr);
     if (bub)
         launch_adj_motion<F>(s, g, h->bp, h->nb, brec.x_start, brec.y_start, fr(rec.vx_end),
                              fr(rec.vy_end), a.f.vx, a.f.vy, a.Lx, a.Ly);

@@ -255,10 +255,12 @@ void launch_adj_pressure_ghosts(hipStream_t s, const GridParams &g, double *L,
                                 const double *el_amp, const double *el_phase, double t,
                                 double *gA, double *gP);
 template <typename F>
+// win (may be null = the whole frame): the cells to evaluate; outside them
+// the evaluation's adjoint inputs must be zero.
 void launch_adj_pressure(hipStream_t s, const GridParams &g, bool first, const AdjField &a,
                          Rect<int64_t> fk, Rect<int64_t> gC, const double *extra, double *Lf,
                          double *LgC, double *LfN, double *LfM, const F *ax, const F *bx,
-                         const F *ay, const F *by);
+                         const F *ay, const F *by, const Window *win);
 void launch_adj_velocity_ghosts(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy);
 template <typename F>
 void launch_adj_velocity(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy,

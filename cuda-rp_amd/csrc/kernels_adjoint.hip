@@ -160,11 +160,11 @@ __global__ __launch_bounds__(256) void k_adj_pressure(
     const double *extra, double *__restrict__ Lf,
     double *__restrict__ LgC, double *__restrict__ LfN, double *LfM,
     const F *__restrict__ ax, const F *__restrict__ bx, const F *__restrict__ ay,
-    const F *__restrict__ by) {
-    const int i = -1 + (int)(blockIdx.x * kAX + threadIdx.x);
-    const int j = -1 + (int)(blockIdx.y * kAY + threadIdx.y);
+    const F *__restrict__ by, Window w) {
+    const int i = w.i0 + (int)(blockIdx.x * kAX + threadIdx.x);
+    const int j = w.j0 + (int)(blockIdx.y * kAY + threadIdx.y);
     const int N = g.N, M = g.M, P = g.pitch;
-    if (i > N + 1 || j > M + 1) return;
+    if (i > w.i1 || j > w.j1) return;
     const double s3 = (double)(F)g.s3;
     const int64_t o = frame_off(i, j, P);
     auto interior = [&](int a, int b) { return a >= 1 && a <= N && b >= 1 && b <= M; };
@@ -214,11 +214,13 @@ __global__ __launch_bounds__(256) void k_adj_pressure(
     }
 }
 
-// The evaluation overwrote p, p0 in the interior: their adjoints are consumed.
-__global__ void k_adj_zero_interior(GridParams g, double *Lp0, double *Lpx, double *Lpy) {
-    const int i = 1 + (int)(blockIdx.x * kAX + threadIdx.x);
-    const int j = 1 + (int)(blockIdx.y * kAY + threadIdx.y);
-    if (i > g.N || j > g.M) return;
+// The evaluation overwrote p, p0 in the interior: their adjoints are consumed
+// (over the interior cells of w).
+__global__ void k_adj_zero_interior(GridParams g, double *Lp0, double *Lpx, double *Lpy,
+                                    Window w) {
+    const int i = w.i0 + (int)(blockIdx.x * kAX + threadIdx.x);
+    const int j = w.j0 + (int)(blockIdx.y * kAY + threadIdx.y);
+    if (i > w.i1 || j > w.j1) return;
     const int64_t o = frame_off(i, j, g.pitch);
     Lp0[o] = 0.0;
     Lpx[o] = 0.0;
@@ -386,88 +388,178 @@ __global__ __launch_bounds__(256) void k_adj_scatter(GridParams g, BubbleParams 
     Ly[idx] += ft * sy;
 }
 
+// FP64 contributions of a workgroup's bubble stencils to a frame array,
+// summed in an LDS tile over the workgroup's bounding box and flushed with
+// one global atomic per touched cell: bubbles are cell-sorted, so a
+// workgroup's stencils cover a small patch, and in dense clouds (hundreds
+// of bubbles per cell) per-contribution global atomics on the same
+// addresses serialise.  A patch larger than the tile falls back to global
+// atomics per contribution.  Every thread of the workgroup calls begin and
+// end (they synchronise); `any` says whether the thread has a stencil.
+constexpr int kAdjTile = 4096;  // cells (32 KB of LDS)
+struct TileAcc {
+    double *tile;
+    int bx0, by0, bw;
+    bool binned;
+};
+__device__ inline TileAcc tile_begin(double *tile, int *box, bool any, int i0, int i1, int j0,
+                                     int j1) {
+    __syncthreads();  // the previous use of box / tile has finished
+    if (threadIdx.x == 0) {
+        box[0] = 0x7fffffff;
+        box[1] = -0x7fffffff;
+        box[2] = 0x7fffffff;
+        box[3] = -0x7fffffff;
+    }
+    __syncthreads();
+    if (any) {
+        atomicMin(&box[0], i0);
+        atomicMax(&box[1], i1);
+        atomicMin(&box[2], j0);
+        atomicMax(&box[3], j1);
+    }
+    __syncthreads();
+    TileAcc t{tile, box[0], box[2], box[1] - box[0] + 1, false};
+    const int bh = box[3] - box[2] + 1;
+    t.binned = t.bw > 0 && bh > 0 && (int64_t)t.bw * bh <= kAdjTile;  // workgroup-uniform
+    if (t.binned) {
+        for (int k = threadIdx.x; k < t.bw * bh; k += blockDim.x) tile[k] = 0.0;
+        __syncthreads();
+    }
+    return t;
+}
+__device__ inline void tile_add(const TileAcc &t, double *glob, int pitch, int i, int j, double v) {
+    if (t.binned)
+        atomicAdd(&t.tile[(j - t.by0) * t.bw + (i - t.bx0)], v);
+    else
+        atomicAdd(&glob[frame_off(i, j, pitch)], v);
+}
+__device__ inline void tile_end(const TileAcc &t, const int *box, double *glob, int pitch) {
+    if (!t.binned) return;
+    __syncthreads();
+    const int n = t.bw * (box[3] - box[2] + 1);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double v = t.tile[k];
+        if (v != 0.0) atomicAdd(&glob[frame_off(t.bx0 + k % t.bw, t.by0 + k / t.bw, pitch)], v);
+    }
+}
+
 // Adjoint of the gather PL_p = sum p0 w of iteration k (p0 = the previous
-// evaluation's pressure): FP64 atomics into Lp0, position adjoints.
+// evaluation's pressure): FP64 atomics into Lp0 (through the workgroup's
+// LDS tile), position adjoints.
 template <typename F>
 __global__ __launch_bounds__(256) void k_adj_gather(GridParams g, BubbleParams bp,
                                                      BubbleArrays b, Rect<F> p0,
                                                      const double *lPL, const double *lPL_extra,
                                                      double *Lp0, double *Lx, double *Ly) {
+    __shared__ double tile[kAdjTile];
+    __shared__ int box[4];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= b.n) return;
-    const double lam = lPL[idx] + (lPL_extra ? lPL_extra[idx] : 0.0);
-    const double x = b.x[idx], y = b.y[idx];
-    const int ibnx = bin_idx(x, g.rdx, 0.5), ibny = bin_idx(y, g.rdy, 0.5);
-    const int nb = bp.mbe - bp.mbs + 1;
+    const bool valid = idx < b.n;
+    double lam = 0.0, x = 0.0, y = 0.0;
+    int ibnx = 0, ibny = 0;
+    if (valid) {
+        lam = lPL[idx] + (lPL_extra ? lPL_extra[idx] : 0.0);
+        x = b.x[idx];
+        y = b.y[idx];
+        ibnx = bin_idx(x, g.rdx, 0.5);
+        ibny = bin_idx(y, g.rdy, 0.5);
+    }
+    const int i0 = max(ibnx + bp.mbs, 0), i1 = min(ibnx + bp.mbe, g.N + 1);
+    const int j0 = max(ibny + bp.mbs, 0), j1 = min(ibny + bp.mbe, g.M + 1);
+    const bool any = valid && lam != 0.0 && i0 <= i1 && j0 <= j1;
+    const TileAcc t = tile_begin(tile, box, any, i0, i1, j0, j1);
     double sx = 0.0, sy = 0.0;
-    for (int a = 0; a < nb; a++) {
-        const int i = ibnx + bp.mbs + a;
-        const double wx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
-        const double dwx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
-        for (int c = 0; c < nb; c++) {
-            const int j = ibny + bp.mbs + c;
-            if (i >= 0 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
+    if (valid) {
+        for (int i = i0; i <= i1; i++) {
+            const double wx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+            const double dwx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+            for (int j = j0; j <= j1; j++) {
                 const double wy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
                 const double dwy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
-                const int64_t o = frame_off(i, j, g.pitch);
-                if (lam != 0.0) atomicAdd(&Lp0[o], lam * wx * wy);
+                if (any) tile_add(t, Lp0, g.pitch, i, j, lam * wx * wy);
                 const double p = (double)p0.at(i, j);
                 sx += p * dwx * wy;
                 sy += p * wx * dwy;
             }
         }
+        Lx[idx] += lam * sx;
+        Ly[idx] += lam * sy;
     }
-    Lx[idx] += lam * sx;
-    Ly[idx] += lam * sy;
+    tile_end(t, box, Lp0, g.pitch);
 }
 
 // Adjoint of k_motion: x' = x + dt vLx(x, y; vx), y' = y + dt vLy(x, y; vy)
 // at the step's start positions (xs, ys); Lx, Ly hold dJ/dx', dJ/dy' and
-// become dJ/dx, dJ/dy.
+// become dJ/dx, dJ/dy.  The node adjoints Lvx, Lvy are summed through the
+// workgroup's LDS tile, one array after the other.
 template <typename F>
 __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams bp, int n,
                                                      const double *xs, const double *ys,
                                                      Rect<F> vx, Rect<F> vy, double *Lvx,
                                                      double *Lvy, double *Lx, double *Ly) {
+    __shared__ double tile[kAdjTile];
+    __shared__ int box[4];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    const double x = xs[idx], y = ys[idx];
-    const double lx = Lx[idx], ly = Ly[idx], dt = g.dt;
-    const int ibmx = bin_idx(x, g.rdx, 1.0), ibmy = bin_idx(y, g.rdy, 1.0);
-    const int ibnx = bin_idx(x, g.rdx, 0.5), ibny = bin_idx(y, g.rdy, 0.5);
+    const bool valid = idx < n;
+    double x = 0.0, y = 0.0, lx = 0.0, ly = 0.0;
+    int ibmx = 0, ibmy = 0, ibnx = 0, ibny = 0;
+    const double dt = g.dt;
+    if (valid) {
+        x = xs[idx];
+        y = ys[idx];
+        lx = Lx[idx];
+        ly = Ly[idx];
+        ibmx = bin_idx(x, g.rdx, 1.0);
+        ibmy = bin_idx(y, g.rdy, 1.0);
+        ibnx = bin_idx(x, g.rdx, 0.5);
+        ibny = bin_idx(y, g.rdy, 0.5);
+    }
     double dvx_dx = 0.0, dvx_dy = 0.0, dvy_dx = 0.0, dvy_dy = 0.0;
-    for (int i = ibmx + bp.nbs; i <= ibmx + bp.nbe; i++) {
-        const double Dx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
-        const double dDx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
-        for (int j = ibny + bp.mbs; j <= ibny + bp.mbe; j++) {
-            if (i >= -1 && i <= g.N + 1 && j >= 0 && j <= g.M + 1) {
-                const double Dy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
-                const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
-                const int64_t o = frame_off(i, j, g.pitch);
-                if (lx != 0.0) atomicAdd(&Lvx[o], dt * lx * Dx * Dy);
-                const double v = (double)vx.at(i, j);
-                dvx_dx += v * dDx * Dy;
-                dvx_dy += v * Dx * dDy;
+    {  // vx nodes: columns ibm + [nbs, nbe] in [-1, N+1], rows ibn + [mbs, mbe] in [0, M+1]
+        const int i0 = max(ibmx + bp.nbs, -1), i1 = min(ibmx + bp.nbe, g.N + 1);
+        const int j0 = max(ibny + bp.mbs, 0), j1 = min(ibny + bp.mbe, g.M + 1);
+        const bool any = valid && lx != 0.0 && i0 <= i1 && j0 <= j1;
+        const TileAcc t = tile_begin(tile, box, any, i0, i1, j0, j1);
+        if (valid)
+            for (int i = i0; i <= i1; i++) {
+                const double Dx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+                const double dDx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+                for (int j = j0; j <= j1; j++) {
+                    const double Dy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
+                    const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
+                    if (any) tile_add(t, Lvx, g.pitch, i, j, dt * lx * Dx * Dy);
+                    const double v = (double)vx.at(i, j);
+                    dvx_dx += v * dDx * Dy;
+                    dvx_dy += v * Dx * dDy;
+                }
             }
-        }
+        tile_end(t, box, Lvx, g.pitch);
     }
-    for (int i = ibnx + bp.mbs; i <= ibnx + bp.mbe; i++) {
-        const double Dx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
-        const double dDx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
-        for (int j = ibmy + bp.nbs; j <= ibmy + bp.nbe; j++) {
-            if (i >= 0 && i <= g.N + 1 && j >= -1 && j <= g.M + 1) {
-                const double Dy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
-                const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
-                const int64_t o = frame_off(i, j, g.pitch);
-                if (ly != 0.0) atomicAdd(&Lvy[o], dt * ly * Dx * Dy);
-                const double v = (double)vy.at(i, j);
-                dvy_dx += v * dDx * Dy;
-                dvy_dy += v * Dx * dDy;
+    {  // vy nodes: columns ibn + [mbs, mbe] in [0, N+1], rows ibm + [nbs, nbe] in [-1, M+1]
+        const int i0 = max(ibnx + bp.mbs, 0), i1 = min(ibnx + bp.mbe, g.N + 1);
+        const int j0 = max(ibmy + bp.nbs, -1), j1 = min(ibmy + bp.nbe, g.M + 1);
+        const bool any = valid && ly != 0.0 && i0 <= i1 && j0 <= j1;
+        const TileAcc t = tile_begin(tile, box, any, i0, i1, j0, j1);
+        if (valid)
+            for (int i = i0; i <= i1; i++) {
+                const double Dx = wdelta(bp.dcx, bp.dcy, i, g.dx, x);
+                const double dDx = wdelta_db(bp.dcx, bp.dcy, i, g.dx, x);
+                for (int j = j0; j <= j1; j++) {
+                    const double Dy = wdelta(bp.dcx, bp.dcz, j, g.dy, y);
+                    const double dDy = wdelta_db(bp.dcx, bp.dcz, j, g.dy, y);
+                    if (any) tile_add(t, Lvy, g.pitch, i, j, dt * ly * Dx * Dy);
+                    const double v = (double)vy.at(i, j);
+                    dvy_dx += v * dDx * Dy;
+                    dvy_dy += v * Dx * dDy;
+                }
             }
-        }
+        tile_end(t, box, Lvy, g.pitch);
     }
-    Lx[idx] = lx + dt * (lx * dvx_dx + ly * dvy_dx);
-    Ly[idx] = ly + dt * (lx * dvx_dy + ly * dvy_dy);
+    if (valid) {
+        Lx[idx] = lx + dt * (lx * dvx_dx + ly * dvy_dx);
+        Ly[idx] = ly + dt * (lx * dvx_dy + ly * dvy_dy);
+    }
 }
 
 // Adjoint of the RP calls (k_bubble_persistent's BubbleRadiusKernel part).
@@ -559,7 +651,7 @@ __global__ __launch_bounds__(1024) void k_jac_scan(int *bins) {
     }
 }
 
-// One item of k_adj_rp_jac on lane d of its eight.
+// One item of k_adj_rp_jac on lane d of its seven.
 __device__ inline void jac_item(const BubbleParams &bp, const BubbleArrays &b, double dt_mix,
                                 const JacSlot *slots, unsigned item, int d,
                                 const RPFastShared &fs, unsigned *mismatch) {
@@ -576,30 +668,28 @@ __device__ inline void jac_item(const BubbleParams &bp, const BubbleArrays &b, d
                    seed(S.rp[3][idx], 3), S.rp[4][idx], S.rp[5][idx], seed(S.PLm[idx], 6),
                    seed(S.PLn[idx], 5), seed(S.PLp[idx], 4), dt_mix, bp.PL0);
     while (L.remain > 0.0 && L.counter < bp.max_substeps) rp_substep_g(L, fk, fs);
-    if (d < 7) {
-        double *J = S.jac + idx;
-        J[(size_t)d * nb] = L.Rt.d;
-        if (S.last) {
-            J[(size_t)(7 + d) * nb] = L.Rp.d;
-            J[(size_t)(14 + d) * nb] = L.Rn.d;
-            J[(size_t)(21 + d) * nb] = L.d1R.d;
-            J[(size_t)(28 + d) * nb] = L.PG.d;
-        }
-    } else if (S.sub[idx] != L.counter) {
-        atomicAdd(mismatch, 1u);
+    double *J = S.jac + idx;
+    J[(size_t)d * nb] = L.Rt.d;
+    if (S.last) {
+        J[(size_t)(7 + d) * nb] = L.Rp.d;
+        J[(size_t)(14 + d) * nb] = L.Rn.d;
+        J[(size_t)(21 + d) * nb] = L.d1R.d;
+        J[(size_t)(28 + d) * nb] = L.PG.d;
     }
+    if (d == 0 && S.sub[idx] != L.counter) atomicAdd(mismatch, 1u);
 }
 
-// Jacobian columns of the RP map, eight lanes per item (one Picard
-// iteration of one bubble): lane d < 7 runs the sub-step in (value, tangent)
-// arithmetic with the tangent seeded on input d (0..3 start state R, Rn,
-// d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m).  The value part repeats the forward's
-// operations exactly, so each lane takes the forward's sub-steps; lane 7
-// only counts them and counts a difference from the taped count in
-// *mismatch.  Output: jac[c * nb + idx] for c = 7 o + d, o = 0 (R_t) and, in
-// the step's last iteration, o = 1..4 (R', Rn', d1R', PG').  Persistent:
-// each wavefront takes the next eight items of ord until none is left.
-__global__ __launch_bounds__(256) void k_adj_rp_jac(BubbleParams bp, BubbleArrays b, double dt_mix,
+// Jacobian columns of the RP map, seven lanes per item (one Picard
+// iteration of one bubble), nine items per wavefront (lane 63 idles): lane
+// d runs the sub-step in (value, tangent) arithmetic with the tangent seeded
+// on input d (0..3 start state R, Rn, d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m).  The
+// value part repeats the forward's operations exactly, so each lane takes
+// the forward's sub-steps; lane 0 counts a difference from the taped count
+// in *mismatch.  Output: jac[c * nb + idx] for c = 7 o + d, o = 0 (R_t) and,
+// in the step's last iteration, o = 1..4 (R', Rn', d1R', PG').  Persistent:
+// each wavefront takes the next nine items of ord until none is left.
+constexpr int kJacPerWave = 9;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_adj_rp_jac(BubbleParams bp, BubbleArrays b, double dt_mix,
                                                      const JacSlot *slots, const unsigned *ord,
                                                      unsigned n_items, unsigned *head,
                                                      unsigned *mismatch) {
@@ -607,14 +697,15 @@ __global__ __launch_bounds__(256) void k_adj_rp_jac(BubbleParams bp, BubbleArray
     rp_vtab_stage(bp.rp_shared, vtab);
     __syncthreads();
     const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
-    const int lane = threadIdx.x & 63, d = lane & 7;
+    const int lane = threadIdx.x & 63, slot = lane / 7, d = lane - 7 * slot;
     for (;;) {
         unsigned first = 0;
-        if (lane == 0) first = atomicAdd(head, 8u);
+        if (lane == 0) first = atomicAdd(head, (unsigned)kJacPerWave);
         first = __shfl(first, 0);
         if (first >= n_items) break;
-        const unsigned q = first + (lane >> 3);
-        if (q < n_items) jac_item(bp, b, dt_mix, slots, ord[q], d, fs, mismatch);
+        const unsigned q = first + slot;
+        if (slot < kJacPerWave && q < n_items)
+            jac_item(bp, b, dt_mix, slots, ord[q], d, fs, mismatch);
     }
 }
 
@@ -709,17 +800,26 @@ template <typename F>
 void launch_adj_pressure(hipStream_t s, const GridParams &g, bool first, const AdjField &a,
                          Rect<int64_t> fk, Rect<int64_t> gC, const double *extra, double *Lf,
                          double *LgC, double *LfN, double *LfM, const F *ax, const F *bx,
-                         const F *ay, const F *by) {
-    const dim3 blk(kAX, kAY), grid((g.N + 3 + kAX - 1) / kAX, (g.M + 3 + kAY - 1) / kAY);
+                         const F *ay, const F *by, const Window *win) {
+    Window w{-1, g.N + 1, -1, g.M + 1};
+    if (win) {
+        w = Window{std::max(win->i0, -1), std::min(win->i1, g.N + 1), std::max(win->j0, -1),
+                   std::min(win->j1, g.M + 1)};
+        if (w.i0 > w.i1 || w.j0 > w.j1) return;
+    }
+    const dim3 blk(kAX, kAY),
+        grid((w.i1 - w.i0 + 1 + kAX - 1) / kAX, (w.j1 - w.j0 + 1 + kAY - 1) / kAY);
     if (first)
         hipLaunchKernelGGL((k_adj_pressure<F, true>), grid, blk, 0, s, g, a.p0, a.px, a.py, a.pnx,
-                           a.pny, a.vx, a.vy, fk, gC, extra, Lf, LgC, LfN, LfM, ax, bx, ay, by);
+                           a.pny, a.vx, a.vy, fk, gC, extra, Lf, LgC, LfN, LfM, ax, bx, ay, by, w);
     else
         hipLaunchKernelGGL((k_adj_pressure<F, false>), grid, blk, 0, s, g, a.p0, a.px, a.py,
                            a.pnx, a.pny, a.vx, a.vy, fk, gC, extra, Lf, LgC, LfN, LfM, ax, bx, ay,
-                           by);
-    const dim3 gi((g.N + kAX - 1) / kAX, (g.M + kAY - 1) / kAY);
-    hipLaunchKernelGGL(k_adj_zero_interior, gi, blk, 0, s, g, a.p0, a.px, a.py);
+                           by, w);
+    const Window wi{std::max(w.i0, 1), std::min(w.i1, g.N), std::max(w.j0, 1), std::min(w.j1, g.M)};
+    if (wi.i0 > wi.i1 || wi.j0 > wi.j1) return;
+    const dim3 gi((wi.i1 - wi.i0 + 1 + kAX - 1) / kAX, (wi.j1 - wi.j0 + 1 + kAY - 1) / kAY);
+    hipLaunchKernelGGL(k_adj_zero_interior, gi, blk, 0, s, g, a.p0, a.px, a.py, wi);
 }
 
 void launch_adj_velocity_ghosts(hipStream_t s, const GridParams &g, double *Lvx, double *Lvy) {
@@ -786,7 +886,7 @@ void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const Bubble
             nblk = 1;
         per_cu = nblk;
     }
-    const long waves = ((long)n_items + 7) / 8;
+    const long waves = ((long)n_items + kJacPerWave - 1) / kJacPerWave;
     const long blocks = std::min<long>((long)per_cu * cus, (waves + 3) / 4);
     hipLaunchKernelGGL(k_adj_rp_jac, dim3((unsigned)blocks), dim3(256), 0, s, bp, b, dt_mix, slots,
                        ord, n_items, head, mismatch);
@@ -821,7 +921,7 @@ void launch_fill_d(hipStream_t s, double *p, size_t n, double v) {
     template void launch_adj_pressure<F>(hipStream_t, const GridParams &, bool, const AdjField &, \
                                          Rect<int64_t>, Rect<int64_t>, const double *,          \
                                          double *, double *, double *, double *, const F *,     \
-                                         const F *, const F *, const F *);                      \
+                                         const F *, const F *, const F *, const Window *);      \
     template void launch_adj_velocity<F>(hipStream_t, const GridParams &, double *, double *,   \
                                          const F *, Rect<int64_t>, int, const F *, const F *,   \
                                          const F *, const F *, double *, double *);             \
