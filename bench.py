@@ -431,8 +431,9 @@ def vf_accumulation(brp, sim):
 
 def ascii_writer_leg(n=4096):
     """The driver's ASCII field writer (cuda-rp_amd/bin/rawdata_bench): one
-    n x n field of save_ascii text, MB/s with the host's threads and with
-    one (the tool also checks the bytes against the serial ostream text)."""
+    n x n field of save_ascii text, MB/s of the formatting alone and of the
+    whole write, with the host's threads and with one (the tool also checks
+    the bytes against the serial ostream text)."""
     import subprocess
     import tempfile
     exe = os.path.join(ROOT, "cuda-rp_amd", "bin", "rawdata_bench")
@@ -444,8 +445,11 @@ def ascii_writer_leg(n=4096):
     for line in r.stdout.splitlines():
         if line.startswith("threads"):
             t = int(line.split()[1].rstrip(":"))
-            out[f"MB_per_s_{t}_threads"] = float(line.split("=")[1].split()[0])
+            out[f"write_MB_per_s_{t}_threads"] = float(line.split("=")[1].split()[0])
             out["MB_per_field"] = float(line.split(":")[1].split()[0])
+        elif line.startswith("format threads"):
+            t = int(line.split()[2].rstrip(":"))
+            out[f"format_MB_per_s_{t}_threads"] = float(line.split("=")[1].split()[0])
     return out
 
 

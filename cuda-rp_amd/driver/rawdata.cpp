@@ -138,10 +138,15 @@ static inline void put_g(std::string &out, double v) {
     out.append(b, (size_t)(r.ptr - b));
 }
 
-void format_ascii_rows(const Field &f, int i_lo, int X, double dx, double dy, int j_begin,
-                       int j_end, std::string &out, double *mn, double *mx) {
-    std::vector<std::string> xs((size_t)(X - i_lo + 1));
-    for (int i = i_lo; i <= X; i++) put_g(xs[(size_t)(i - i_lo)], (double)i * dx);
+// xs: the x column's text, shared by every block (built once per field: a
+// per-block array above the allocator's mmap threshold is mapped and
+// unmapped by every call, and unmapping stalls every thread of the process).
+// The extrema are kept in locals and stored once: threads updating
+// neighbouring slots per value share a cache line and stop scaling.
+static void format_ascii_rows(const Field &f, const std::vector<std::string> &xs, int i_lo, int X,
+                              double dy, int j_begin, int j_end, std::string &out, double *mn,
+                              double *mx) {
+    double lo = *mn, hi = *mx;
     for (int j = j_begin; j < j_end; j++) {
         std::string y;
         put_g(y, (double)j * dy);
@@ -153,49 +158,100 @@ void format_ascii_rows(const Field &f, int i_lo, int X, double dx, double dy, in
             out += '\t';
             put_g(out, v);
             out += '\n';
-            *mx = std::max(*mx, v);
-            *mn = std::min(*mn, v);
+            hi = std::max(hi, v);
+            lo = std::min(lo, v);
         }
         out += '\n';
     }
+    *mn = lo;
+    *mx = hi;
 }
 
-bool write_ascii_field(const std::string &path, const Field &f, int X, int Y, bool cylindrical,
-                       double dx, double dy, int threads, double *mn, double *mx) {
+// Rows are formatted in blocks of kBlockRows by `threads` worker threads that
+// take blocks in order from a shared counter into a ring of 4 x threads
+// buffers; the calling thread writes the blocks in order as they complete,
+// so the text is produced and written at the same time and a worker waits
+// only when the ring is full.  `fp` null: format only (the text is
+// discarded; for timing).  Returns the bytes produced, or -1 on a write
+// error.
+static long long ascii_pipeline(FILE *fp, const Field &f, int X, int Y, bool cylindrical, double dx,
+                                double dy, int threads, double *mn, double *mx) {
     const int i_lo = cylindrical ? -X : 0;
     const int rows = Y + 1;
     constexpr int kBlockRows = 16;
     const int nblk = (rows + kBlockRows - 1) / kBlockRows;
     threads = std::max(1, std::min(threads, nblk));
-    FILE *fp = std::fopen(path.c_str(), "wb");
-    if (!fp) return false;
-    // Rounds of `threads` consecutive row blocks: formatted in parallel into
-    // buffers that are reused from round to round (so their pages are
-    // touched once), then written in order.
-    std::vector<std::string> buf((size_t)threads);
+    const int ring = 4 * threads;
+    std::vector<std::string> buf((size_t)ring);
+    std::vector<int> ready((size_t)ring, -1);  // block held by each slot, -1 while being formatted
+    std::vector<std::string> xs((size_t)(X - i_lo + 1));
+    for (int i = i_lo; i <= X; i++) put_g(xs[(size_t)(i - i_lo)], (double)i * dx);
+    std::mutex m;
+    std::condition_variable cv_ready, cv_free;
+    int next = 0, written = 0;  // next block to format, blocks written (under m)
     std::vector<double> lo((size_t)threads, 1e300), hi((size_t)threads, -1e300);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back([&, t] {
+            double l = 1e300, h = -1e300;
+            for (;;) {
+                int b;
+                {
+                    std::unique_lock<std::mutex> lk(m);
+                    if (next >= nblk) break;
+                    b = next++;
+                    cv_free.wait(lk, [&] { return b - written < ring; });  // slot b % ring is free
+                }
+                std::string &out = buf[(size_t)(b % ring)];
+                out.clear();
+                const int j0 = b * kBlockRows, j1 = std::min(rows, j0 + kBlockRows);
+                format_ascii_rows(f, xs, i_lo, X, dy, j0, j1, out, &l, &h);
+                {
+                    std::lock_guard<std::mutex> lk(m);
+                    ready[(size_t)(b % ring)] = b;
+                }
+                cv_ready.notify_all();
+            }
+            lo[(size_t)t] = l;
+            hi[(size_t)t] = h;
+        });
+    long long bytes = 0;
     bool ok = true;
-    for (int b0 = 0; b0 < nblk && ok; b0 += threads) {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < threads && b0 + t < nblk; t++)
-            pool.emplace_back([&, t] {
-                const int j0 = (b0 + t) * kBlockRows, j1 = std::min(rows, j0 + kBlockRows);
-                buf[(size_t)t].clear();
-                format_ascii_rows(f, i_lo, X, dx, dy, j0, j1, buf[(size_t)t], &lo[(size_t)t],
-                                  &hi[(size_t)t]);
-            });
-        for (auto &th : pool) th.join();
-        for (int t = 0; t < threads && b0 + t < nblk; t++)
-            ok = ok && std::fwrite(buf[(size_t)t].data(), 1, buf[(size_t)t].size(), fp) ==
-                           buf[(size_t)t].size();
+    for (int b = 0; b < nblk; b++) {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_ready.wait(lk, [&] { return ready[(size_t)(b % ring)] == b; });
+        }
+        const std::string &out = buf[(size_t)(b % ring)];
+        bytes += (long long)out.size();
+        if (fp && ok) ok = std::fwrite(out.data(), 1, out.size(), fp) == out.size();
+        {
+            std::lock_guard<std::mutex> lk(m);
+            ready[(size_t)(b % ring)] = -1;
+            written = b + 1;
+        }
+        cv_free.notify_all();
     }
-    ok = (std::fclose(fp) == 0) && ok;
+    for (auto &th : pool) th.join();
     if (mn)
         for (int t = 0; t < threads; t++) {
             *mn = std::min(*mn, lo[(size_t)t]);
             *mx = std::max(*mx, hi[(size_t)t]);
         }
-    return ok;
+    return ok ? bytes : -1;
+}
+
+bool write_ascii_field(const std::string &path, const Field &f, int X, int Y, bool cylindrical,
+                       double dx, double dy, int threads, double *mn, double *mx) {
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    const bool ok = ascii_pipeline(fp, f, X, Y, cylindrical, dx, dy, threads, mn, mx) >= 0;
+    return (std::fclose(fp) == 0) && ok;
+}
+
+long long format_ascii_field(const Field &f, int X, int Y, bool cylindrical, double dx, double dy,
+                             int threads) {
+    return ascii_pipeline(nullptr, f, X, Y, cylindrical, dx, dy, threads, nullptr, nullptr);
 }
 
 static int writer_threads() {

@@ -65,10 +65,15 @@ class RawWriter {
 std::string step_name(const std::string &dir, const char *prefix, int step, const char *ext);
 // Writes one field as save_ascii's text (x, y, value lines, rows j = 0..Y,
 // columns (cylindrical ? -X : 0)..X with the value at |i|, a blank line per
-// row), formatting row ranges on `threads` threads; the bytes equal the
-// serial ostream output.  mn / mx (may be null) take the running extrema.
+// row), formatting row blocks on `threads` threads while the previous blocks
+// are written; the bytes equal the serial ostream output.  mn / mx (may be
+// null) take the running extrema.
 bool write_ascii_field(const std::string &path, const Field &f, int X, int Y, bool cylindrical,
                        double dx, double dy, int threads, double *mn, double *mx);
+// The same text formatted on `threads` threads and discarded (for timing the
+// formatting apart from the file system); returns its size in bytes.
+long long format_ascii_field(const Field &f, int X, int Y, bool cylindrical, double dx, double dy,
+                             int threads);
 // Writes one float32 SPH scalar record file (SphData::SaveSph layout).
 bool save_sph_file(const std::string &path, int nx, int ny, float dx, float dy, float time,
                    int step, const std::vector<float> &data);

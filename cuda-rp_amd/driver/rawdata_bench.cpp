@@ -4,9 +4,10 @@
 //
 // Writes small fields (plain and cylindrical-mirrored layouts, special values)
 // with 1, 3, 8 and 64 threads and compares the bytes with the serial ostream
-// text of save_ascii (src/output_styles.cpp:182-327), then writes one N x N
-// field with THREADS threads and with one, printing MB/s.  Exit status 0 and a
-// last line "OK" when every comparison matched.
+// text of save_ascii (src/output_styles.cpp:182-327), then formats one N x N
+// field without writing it and writes it, each with THREADS threads and with
+// one, printing MB/s.  Exit status 0 and a last line "OK" when every
+// comparison matched.
 
 #include "rawdata.h"
 #include <chrono>
@@ -62,6 +63,12 @@ int main(int argc, char **argv) {
     f.nx = big + 2; f.ny = big + 2;
     f.v.resize((size_t)f.nx * f.ny);
     for (auto &v : f.v) v = 1.0e6 * u(rng);
+    for (int t : {threads, 1}) {  // formatting alone (no file system)
+        const auto t0 = std::chrono::steady_clock::now();
+        const long long n = format_ascii_field(f, big, big, false, 1e-4, 1e-4, t);
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("format threads %d: %.1f MB in %.2f s = %.0f MB/s\n", t, n / 1e6, s, n / 1e6 / s);
+    }
     for (int t : {threads, 1}) {
         const std::string p = dir + "/big.txt";
         const auto t0 = std::chrono::steady_clock::now();
