@@ -800,7 +800,7 @@ void rp_grid_query(int device, int *cus, int *per_cu) {
 
 bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold, int cus) {
     const double lanes = 256.0 * cus;  // lanes of the lean grid
-    return sub_max >= 2 * extreme_threshold && sub_per_launch < 0.35 * lanes * sub_max;
+    return sub_max >= 2 * extreme_threshold && sub_per_launch < 0.6 * lanes * sub_max;
 }
 
 int bubble_step_blocks(int n_bubbles, bool lean, int cus, int per_cu) {
