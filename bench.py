@@ -18,10 +18,13 @@ Every run first takes PREROLL = 300 untimed steps, which bring the wavefront
 onto the bubble clouds (the focus is ~260 steps from the source) so that the
 timed window is inside the cavitating regime the benchmark is about; then W
 further untimed warm-up steps, then K timed steps.  Beside the headline: the
-all-FP64 run of the same window, mixed-vs-FP64 window statistics, the error
-of the mixed run against the FP64 oracle seeded with the GPU's state at the
-window start, the 2048^2 workload, the temperature solve, the adjoint
-workload and the CPU baseline.  One JSON line on stdout; progress on stderr.
+all-FP64 run of the same window, mixed-vs-FP64 window statistics, the
+void-fraction accumulation schemes on the FP64 run's radii, the error of the
+mixed run against the FP64 oracle seeded with the GPU's state at the window
+start, the CPU baseline, the ASCII writer, the 2048^2 workload, the
+temperature solve and the 1024^2 adjoint workload.  --at-scale runs only the
+4096^2 adjoint workload (8000 steps, a few minutes).  One JSON line on
+stdout; progress on stderr.
 """
 import argparse
 import json

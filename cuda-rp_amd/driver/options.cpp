@@ -66,8 +66,8 @@ bool to_bool(std::string s, bool *v) {
 
 // The registered options: file_opts, generic_opts and hidden_opts of
 // src/bubbles.cpp:66-136 (note "debug p_min" is registered with a space, so
-// no configuration line can ever set it), then the keys of this build's
-// extensions E1-E4 and run controls under [extensions].
+// no configuration line can ever set it), then the keys of the
+// library's extensions E1-E4 and run controls under [extensions].
 const std::vector<Opt> &file_options() {
     static const std::vector<Opt> v = {
         {"grid.x", INT, I32(grid_x), ""},
@@ -129,7 +129,7 @@ const std::vector<Opt> &file_options() {
         {"debug.v_min", DOUBLE, [](Options &, const std::string &) {}, ""},
         {"debug.v_max", DOUBLE, [](Options &, const std::string &) {}, ""},
         {"debug.bubbles", BOOL, [](Options &o, const std::string &v) { to_bool(v, &o.dbg_bubbles); }, " (=1)"},
-        // extensions of this build (ignored by the reference as unregistered)
+        // the library's extensions (ignored by the reference as unregistered)
         {"extensions.precision", STRING,
          [](Options &o, const std::string &v) { o.cfg.precision = (v == "mixed" || v == "1") ? BRP_PREC_MIXED : BRP_PREC_FP64; },
          " fp64|mixed (=fp64)"},
