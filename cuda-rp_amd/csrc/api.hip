@@ -1336,6 +1336,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
     }
     std::vector<JacSlot> slots_h;
     std::vector<std::vector<const double *>> jk(K);
+    const int jac_per_cu = bub ? adj_rp_jac_per_cu() : 1;
 
     // reverse sweep: per segment, the steps are recomputed from the segment's
     // checkpoint with their intermediates recorded, then reversed
@@ -1392,7 +1393,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
                 HIPCHK(h, hipMemcpyAsync(a.slots, slots_h.data(), sizeof(JacSlot) * slots_h.size(),
                                          hipMemcpyHostToDevice, s));
                 launch_adj_rp_jacobians(s, h->bp, h->b, h->g.dt, a.slots, (int)slots_h.size(), a.ord,
-                                        a.bins, a.head, a.mismatch, h->cus);
+                                        a.bins, a.head, a.mismatch, h->cus, jac_per_cu);
             }
             for (int st = hi; st >= lo; st--)
                 adjoint_step<F>(h, recs[st - s0], use_tape ? tape[st - 1] : recs[st - s0],

@@ -295,7 +295,9 @@ constexpr int kJacCols = 7, kJacColsLast = 35;
 // counted in *mismatch.
 void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const BubbleArrays &b,
                              double dt_mix, const JacSlot *slots, int n_slots, unsigned *ord,
-                             int *bins, unsigned *head, unsigned *mismatch, int cus);
+                             int *bins, unsigned *head, unsigned *mismatch, int cus, int per_cu);
+// Workgroups of the Jacobian kernel one compute unit holds (its occupancy).
+int adj_rp_jac_per_cu();
 // The transpose of one iteration's RP map: LRe (the adjoint of the step's
 // final RP state) only for the last iteration, else null.
 void launch_adj_rp_apply(hipStream_t s, int nb, const double *jac, const double *lRt,

@@ -864,9 +864,17 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
                        vx, vy, Lvx, Lvy, Lx, Ly);
 }
 
+int adj_rp_jac_per_cu() {
+    int nblk = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k_adj_rp_jac, 256, 0) != hipSuccess ||
+        nblk < 1)
+        nblk = 1;
+    return nblk;
+}
+
 void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const BubbleArrays &b,
                              double dt_mix, const JacSlot *slots, int n_slots, unsigned *ord,
-                             int *bins, unsigned *head, unsigned *mismatch, int cus) {
+                             int *bins, unsigned *head, unsigned *mismatch, int cus, int per_cu) {
     if (b.n <= 0 || n_slots <= 0) return;
     const unsigned n_items = (unsigned)((long)n_slots * b.n);
     const unsigned tiles = (n_items + kJacTile - 1) / kJacTile;
@@ -877,15 +885,6 @@ void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const Bubble
     hipLaunchKernelGGL(k_jac_scan, dim3(1), dim3(1024), 0, s, bins);
     hipLaunchKernelGGL(k_jac_rank<true>, dim3(tiles), dim3(256), 0, s, slots, b.n, n_items, bins,
                        ord);
-    static int per_cu = 0;
-    if (!per_cu) {
-        int nblk = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k_adj_rp_jac, 256, 0) !=
-                hipSuccess ||
-            nblk < 1)
-            nblk = 1;
-        per_cu = nblk;
-    }
     const long waves = ((long)n_items + kJacPerWave - 1) / kJacPerWave;
     const long blocks = std::min<long>((long)per_cu * cus, (waves + 3) / 4);
     hipLaunchKernelGGL(k_adj_rp_jac, dim3((unsigned)blocks), dim3(256), 0, s, bp, b, dt_mix, slots,
