@@ -322,8 +322,10 @@ __device__ inline crpl *rpl(const RPLaunch *p) {
 }
 
 // Finished lanes a wavefront collects before it leaves the sub-step loop to
-// write them back and refill them.
-constexpr int kRefillBatch = 24;
+// write them back and refill them (4096^2 clustered on the lean grid, ms per
+// step: 16 24.5, 24 23.3, 32 23.0, 40 23.2, 48 24.2; 2048^2 equal within 1 %
+// from 24 to 48).
+constexpr int kRefillBatch = 32;
 
 template <typename F, bool FAST>
 __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, int max_substeps,
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         // wave with a long chain among short bubbles pay the write-back,
         // scatter, pressure gather and set-up nearly every iteration
         // (measured, as ms per step: 2048^2 16.9 -> 16.0, 4096^2 clustered
-        // 46.1 -> 29.2 with a batch of 24; 16-32 are equal within 2 %).
+        // 46.1 -> 29.2 with a batch of 24 on the full grid).
         bool fin, capped;
         double remain;
         while (true) {
