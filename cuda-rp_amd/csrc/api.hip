@@ -434,6 +434,7 @@ int alloc_bubbles(brp_handle *h, int n) {
         if ((rc = dalloc(h, (void **)a, bytes))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.sub, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.order, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->b.key, sizeof(int) * (size_t)std::max(n, 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->b.order_tmp, sizeof(int) * (size_t)std::max(n, 1))) ||
         (rc = dalloc(h, (void **)&h->b.sort_hist, sizeof(int) * 2048)))
         return rc;
@@ -937,8 +938,18 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 HIPCHK(h, hipEventRecord(h->ev[7], s));
                 const StepRecord *tin = ctl ? ctl->tape_in : nullptr;
                 const bool replay = h->sched_mode == 2;
-                if (!tin && !replay)
-                    launch_order_partition(s, h->b, h->bp.heavy_threshold, h->bp.extreme_threshold);
+                if (!tin && !replay) {
+                    // the first iteration's queue from the start state (the
+                    // previous step's counts do not find this step's longest
+                    // chains), the later ones from the previous iteration's counts
+                    const int *key = h->b.sub;
+                    if (loop == 1) {
+                        launch_first_keys(s, h->b, h->rp_cur, h->bp.extreme_threshold);
+                        key = h->b.key;
+                    }
+                    launch_order_partition(s, h->b, key, h->bp.heavy_threshold,
+                                           h->bp.extreme_threshold);
+                }
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
                 if (tin) {  // replay: the void fraction from the taped radii, no RP
                     launch_scatter_radii(s, h->g, h->bp, h->b, tin->Rt_it[loop], h->fgb[target], h->sc_d);

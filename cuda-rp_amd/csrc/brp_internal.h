@@ -91,6 +91,7 @@ struct BubbleArrays {
     double *vx, *vy;      // v_B (= v_L)
     int *sub;             // sub-steps of the last RP call, per bubble
     int *order;           // work-queue order: bubbles with many sub-steps first
+    int *key;             // work-queue keys of a step's first Picard iteration
     int *order_tmp;       // partition scratch: the extreme class before its sort
     int *sort_hist;       // partition scratch: 2048 bin counts / offsets of that sort
     int *part;            // partition scratch: per-block extreme and heavy offsets + totals
@@ -194,7 +195,10 @@ int bubble_step_blocks(int n_bubbles, bool lean, int cus, int per_cu);
 // longest chain of the last launch is long and the sub-steps per lane of a
 // one-workgroup-per-CU grid are few against it.
 bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold, int cus);
-void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext);
+// Queue keys of a step's first Picard iteration (predicted sub-step counts).
+void launch_first_keys(hipStream_t s, const BubbleArrays &b, int cur, int ext);
+void launch_order_partition(hipStream_t s, const BubbleArrays &b, const int *key, int threshold,
+                            int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                             const BubbleArrays &b, int64_t *fg_target, StepScalars *sc);
 void launch_cyl_fold(hipStream_t s, const GridParams &g, int64_t *fg);

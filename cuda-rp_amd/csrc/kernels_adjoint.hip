@@ -658,10 +658,7 @@ __device__ inline void jac_item(const BubbleParams &bp, const BubbleArrays &b, d
     const unsigned nb = (unsigned)b.n;
     const unsigned sl = item / nb, idx = item - sl * nb;
     const JacSlot &S = slots[sl];
-    const double R0 = b.R0[idx];
-    const double PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
-    const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
-                         bp.T0 / (PG0 * R03)};
+    const RPFastConst fk = rp_fast_const(b.R0[idx], b.PG0[idx], b.calpha[idx], bp.T0, fs);
     auto seed = [&](double v, int k) { return dd(v, d == k ? 1.0 : 0.0); };
     RPLaneF<dd> L;
     rp_begin_g<dd>(L, seed(S.rp[0][idx], 0), seed(S.rp[1][idx], 1), seed(S.rp[2][idx], 2),

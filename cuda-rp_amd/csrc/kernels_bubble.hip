@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         const BubbleParams &bp = rpl(kp)->bp;
         sh = RPShared{bp.PL0, bp.T0, bp.gam, bp.sig, bp.mu, bp.rho, bp.K0, bp.dt0, bp.pi};
     }
-    RPFastConst fk{0, 0, 0, 0, 0, 0, 0};
+    RPFastConst fk{0, 0, 0, 0, 0, 0, 0, 0};
     // The E longest bubbles (the queue's first entries) run one per workgroup
     // in lane 0 of its first wavefront, alone: a wavefront issues the same
     // instructions whether one lane or 64 are active, so a lone lane skips
@@ -443,9 +443,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
             b.PL[plp][idx] = PL;
             const double R0 = b.R0[idx];
             k = RPConst{R0, R0 * R0 * R0, b.PG0[idx], b.calpha[idx]};
-            if (FAST)
-                fk = RPFastConst{R0,        1.0 / R0,          k.PG0, 1.0 / k.PG0, k.calpha,
-                                 k.PG0 * k.R03, bp.T0 / (k.PG0 * k.R03)};
+            if (FAST) fk = rp_fast_const(R0, k.PG0, k.calpha, bp.T0, fsh);
             if (FAST) {
                 rp_begin_g(LF, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
                            b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], PL,
@@ -557,9 +555,7 @@ __global__ __launch_bounds__(256) void k_rp_record(const RPLaunch *kp, int cur, 
     const BubbleArrays &b = P.b;
     if (idx >= b.n) return;
     const RPFastShared fs = rp_shared_load(P.bp.rp_shared, vtab);
-    const double R0 = b.R0[idx], PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
-    const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
-                         P.bp.T0 / (PG0 * R03)};
+    const RPFastConst fk = rp_fast_const(b.R0[idx], b.PG0[idx], b.calpha[idx], P.bp.T0, fs);
     RPLaneF<double> L;
     rp_begin_g(L, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
                b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], b.PL[plp][idx],
@@ -605,9 +601,7 @@ __global__ __launch_bounds__(256) void k_rp_replay(const RPLaunch *kp, int cur, 
         const double PL = gather_p0(g, bp, p0, x, y, ibnx, ibny);
         b.PL[plp][idx] = PL;
         const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
-        const double R0 = b.R0[idx], PG0 = b.PG0[idx], R03 = R0 * R0 * R0;
-        const RPFastConst fk{R0, 1.0 / R0, PG0, 1.0 / PG0, b.calpha[idx], PG0 * R03,
-                             bp.T0 / (PG0 * R03)};
+        const RPFastConst fk = rp_fast_const(b.R0[idx], b.PG0[idx], b.calpha[idx], bp.T0, fs);
         RPLaneF<double> L;
         rp_begin_g(L, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
                    b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], PL, g.dt,
@@ -1002,21 +996,48 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int *sub, const int 
     }
 }
 
-void launch_order_partition(hipStream_t s, const BubbleArrays &b, int threshold, int ext) {
+// Queue keys of a step's first Picard iteration.  In a cavitating step the
+// longest RP chains belong to bubbles that are already collapsing fast when
+// the step starts, and they are other bubbles every step, so the previous
+// step's counts do not find them.  A bubble shrinking faster than
+// kCollapseSpeed gets the key 10 (-dR/dt) R / R0, an estimate of its
+// sub-steps (on the 4096^2 clustered workload the median count is about ten
+// times that score up to 200 m/s, and ordered by it the 16 longest chains of
+// a step sit within its first 300 queue entries and the 4096 longest within
+// the first 35 000 of 4.2 million); every other bubble keeps its previous
+// count, capped below the extreme class.
+constexpr double kCollapseSpeed = 25.0;  // m/s
+
+__global__ __launch_bounds__(256) void k_first_keys(BubbleArrays b, int cur, int ext) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= b.n) return;
+    const double v = b.d1R[cur][idx];
+    int k = min(b.sub[idx], ext - 1);
+    if (v < -kCollapseSpeed) k = (int)fmin(-10.0 * v * (b.R[cur][idx] / b.R0[idx]), 1.0e6);
+    b.key[idx] = k;
+}
+
+void launch_first_keys(hipStream_t s, const BubbleArrays &b, int cur, int ext) {
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_first_keys, dim3((b.n + 255) / 256), dim3(256), 0, s, b, cur, ext);
+}
+
+void launch_order_partition(hipStream_t s, const BubbleArrays &b, const int *key, int threshold,
+                            int ext) {
     if (b.n <= 0) return;
     const int nblk = (b.n + 255) / 256;
     int *cntE = b.part, *cntH = b.part + (nblk + 1);
-    hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
+    hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, key, b.n, ext, threshold,
                        cntE, cntH, b.sort_hist);
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntE, nblk);
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntH, nblk);
-    hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, b.sub, b.n, ext, threshold,
+    hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, key, b.n, ext, threshold,
                        cntE, cntH, nblk, b.order, b.order_tmp);
     const int sblk = std::min(1024, nblk);
-    hipLaunchKernelGGL(k_sort_hist, dim3(sblk), dim3(256), 0, s, b.sub, cntE + nblk, b.order_tmp,
+    hipLaunchKernelGGL(k_sort_hist, dim3(sblk), dim3(256), 0, s, key, cntE + nblk, b.order_tmp,
                        b.sort_hist);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, b.sort_hist);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(sblk), dim3(256), 0, s, b.sub, cntE + nblk,
+    hipLaunchKernelGGL(k_sort_scatter, dim3(sblk), dim3(256), 0, s, key, cntE + nblk,
                        b.order_tmp, b.sort_hist, b.order);
 }
 

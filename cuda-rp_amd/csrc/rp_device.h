@@ -212,8 +212,8 @@ __device__ inline void rp_end(const RPLane &L, double dt_mix, RPState *out, doub
 }
 
 // ---- constants of the shortened-chain sub-step (rp_fast.h) ---------------
-struct RPFastConst {   // per bubble
-    double R0, iR0, PG0, iPG0, calpha, PG0R03, T0_PG0R03;
+struct RPFastConst {   // per bubble (rp_fast_const)
+    double iR0, PG0, iPG0, calpha, PG0R03, T0_PG0R03, A0, B0;
 };
 struct RPFastShared {  // shared by all bubbles
     double PL0, gam, mu, rho, irho, dt0, g3, k3, C0, two_sig, four_mu, mu2, m2sig_rho,

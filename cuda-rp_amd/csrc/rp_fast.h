@@ -89,6 +89,15 @@ __device__ inline dd rsqrt_(dd x) {
     const double r = drsqrt(x.v);
     return dd(r, -0.5 * r * r * r * x.d);
 }
+// Per-bubble constants of the sub-step.  A0 and B0 are the Omega iteration's
+// (rp_substep_g): 3 gam rho PG0 R0^3 and -3 gam PG0 R0^3 / 4.
+__device__ inline RPFastConst rp_fast_const(double R0, double PG0, double calpha, double T0,
+                                            const RPFastShared &s) {
+    const double PG0R03 = PG0 * (R0 * R0 * R0);
+    return RPFastConst{1.0 / R0,      PG0,    1.0 / PG0, calpha, PG0R03, T0 / PG0R03,
+                       (s.g3 * s.rho) * PG0R03, (-0.25 * s.g3) * PG0R03};
+}
+
 // Constant (scalar-loadable) memory.
 typedef __attribute__((address_space(4))) const double cdouble;
 
@@ -191,9 +200,11 @@ __device__ inline bool small_alpha(T s) {
 // Upsilon(alpha) = 3 gam / (1 + k3 U) (bubble_CUDA_kernel.cuh:296-311) from
 // s and is = 1/s, U = (alpha coth alpha - 1) / alpha^2: the table up to
 // s = 20; beyond it coth alpha = 1 to below an ulp, so
-// U = (alpha - 1) / alpha^2 = (is / 2) (1 + i (is - 1)).
+// U = (alpha - 1) / alpha^2 = (is / 2) (1 + i (is - 1)).  Returned as
+// conj(Upsilon) / (3 gam) = conj(1 / (1 + k3 U)): the caller's constants carry
+// the factor 3 gam and the sign of the imaginary part.
 template <typename T>
-__device__ inline cx<T> upsilon_g(T s, T is, double g3, double k3, const double *vtab) {
+__device__ inline cx<T> upsilon_g(T s, T is, double k3, const double *vtab) {
     cx<T> u;
     if (val(s) > 20.0) {
         const T h = 0.5 * is;
@@ -207,8 +218,8 @@ __device__ inline cx<T> upsilon_g(T s, T is, double g3, double k3, const double 
         u = v_table(s, vtab);
     }
     const T dr = fma_(T(k3), u.re, T(1.0)), di = k3 * u.im;
-    const T f = g3 * rcp_(fma_(dr, dr, di * di));  // g3 = 3 gam
-    return cx<T>{dr * f, -(di * f)};
+    const T f = rcp_(fma_(dr, dr, di * di));
+    return cx<T>{dr * f, di * f};
 }
 
 // solveLp (bubble_CUDA_kernel.cuh:314-320) as solvePG uses it
@@ -320,25 +331,28 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     L.Rp = Rp;
     L.Rn = Rn;
     L.d1R = d1R;
-    // solveOmegaN at (PGn, Rn) (bubble_CUDA_kernel.cuh:263-289)
-    const T Rx = Rn * k.iR0;
-    const T coef1 = k.PG0 * rcp_(PGn * (Rx * Rx * Rx));
-    const T coef2 = PGn * s.rho;
-    const T coef3 = 4.0 * (iRn * iRn);
+    // solveOmegaN at (PGn, Rn) (bubble_CUDA_kernel.cuh:263-289).  PGn cancels
+    // from the iteration: with coef1 = PG0 / (PGn Rx^3) and coef2 = PGn rho,
+    // Upsilon.re coef1 coef2 = (Upsilon.re / 3 gam) A and
+    // Upsilon.im coef1 PGn / 4 = (-Upsilon.im / 3 gam) B with A = A0 / Rn^3,
+    // B = B0 / Rn^3 (rp_fast_const), so the iteration depends on Rn alone.
+    const T iRn2 = iRn * iRn;
+    const T iRn3 = iRn2 * iRn;
+    const T A = k.A0 * iRn3, B = k.B0 * iRn3;
+    const T coef3 = 4.0 * iRn2;
     const T value1 = s.m2sig_rho * iRn;
     const T scale = iRn * s.irho;
     const T iscale = Rn * s.rho;
-    T eta = fma_(s.g3 * coef1, coef2, fma_(T(-s.mu2), coef3, value1));
+    T eta = A + fma_(T(-s.mu2), coef3, value1);
     T om, iom;
     omega_g(eta, scale, iscale, &om, &iom);
     const T ca = k.calpha * iRn;
     for (int it = 0; it < 3; it++) {
         const T x = ca * om;
         const T isa = rsqrt_(x);
-        const cx<T> ups = upsilon_g(x * isa, isa, s.g3, s.k3, s.vtab);
-        const T ur = ups.re * coef1, ui = ups.im * coef1;
-        const T mu_eff = fma_(ui * PGn, 0.25 * iom, T(s.mu));  // mu + ui PG / (4 omega)
-        eta = fma_(ur, coef2, fma_(-(mu_eff * mu_eff), coef3, value1));
+        const cx<T> ups = upsilon_g(x * isa, isa, s.k3, s.vtab);
+        const T mu_eff = fma_(ups.im * B, iom, T(s.mu));  // mu + ui PG / (4 omega)
+        eta = fma_(ups.re, A, fma_(-(mu_eff * mu_eff), coef3, value1));
         omega_g(eta, scale, iscale, &om, &iom);
     }
     const T x = ca * om;
