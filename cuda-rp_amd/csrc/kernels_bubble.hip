@@ -468,6 +468,15 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         // 46.1 -> 29.2 with a batch of 24 on the full grid).
         bool fin, capped;
         double remain;
+        if (FAST && solo) {
+            // a dedicated wavefront: lane 0 runs its bubble to the sync time
+            // without the batch bookkeeping of the shared loop below
+            if (active)
+                while (LF.remain > 0.0 && LF.counter < max_substeps) rp_substep_g(LF, fk, fsh);
+            capped = active && LF.counter >= max_substeps;
+            remain = active ? LF.remain : 0.0;
+            fin = active;
+        } else
         while (true) {
             capped = false;
             remain = 0.0;
