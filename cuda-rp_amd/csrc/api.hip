@@ -1143,6 +1143,7 @@ struct AdjState {
     // RP Jacobians of a run of steps (k_adj_rp_jac): columns, iteration
     // descriptors, longest-first order and its bins and queue head
     double *jac = nullptr;
+    double *jac_buf = nullptr;  // reverse-mode lane states of k_adj_rp_jac
     JacSlot *slots = nullptr;
     unsigned *ord = nullptr, *head = nullptr;
     int *bins = nullptr;
@@ -1342,12 +1343,13 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
         if ((rc = mem.get(h, &a.jac, std::max<size_t>(jac_cap, 1))) ||
             (rc = mem.get(h, &a.slots, std::max<size_t>(slot_cap, 1))) ||
             (rc = mem.get(h, &a.ord, std::max<size_t>(item_cap, 1))) ||
-            (rc = mem.get(h, &a.head, 1)) || (rc = mem.get(h, &a.bins, 4096)))
+            (rc = mem.get(h, &a.head, 2)) || (rc = mem.get(h, &a.bins, kAdjJacBins)))
             return rc;
     }
     std::vector<JacSlot> slots_h;
     std::vector<std::vector<const double *>> jk(K);
-    const int jac_per_cu = bub ? adj_rp_jac_per_cu() : 1;
+    const int jac_blocks = bub ? adj_rp_jac_per_cu() * h->cus : 1;
+    if (bub && (rc = mem.get(h, &a.jac_buf, adj_rp_jac_buf_doubles(jac_blocks)))) return rc;
 
     // reverse sweep: per segment, the steps are recomputed from the segment's
     // checkpoint with their intermediates recorded, then reversed
@@ -1404,7 +1406,7 @@ int adjoint_impl(brp_handle *h, const brp_objective &ob, double *grad_amp, doubl
                 HIPCHK(h, hipMemcpyAsync(a.slots, slots_h.data(), sizeof(JacSlot) * slots_h.size(),
                                          hipMemcpyHostToDevice, s));
                 launch_adj_rp_jacobians(s, h->bp, h->b, h->g.dt, a.slots, (int)slots_h.size(), a.ord,
-                                        a.bins, a.head, a.mismatch, h->cus, jac_per_cu);
+                                        a.bins, a.head, a.mismatch, a.jac_buf, jac_blocks);
             }
             for (int st = hi; st >= lo; st--)
                 adjoint_step<F>(h, recs[st - s0], use_tape ? tape[st - 1] : recs[st - s0],

@@ -293,15 +293,20 @@ struct JacSlot {
     int last;  // the step's last Picard iteration
 };
 constexpr int kJacCols = 7, kJacColsLast = 35;
-// The Jacobians of n_slots iterations (slots in device memory), ordered
-// longest-first (ord: n_slots x nb indices, bins: 4096 ints, head: one
-// counter); replays whose sub-step count differs from the taped one are
+// The Jacobians of n_slots iterations (slots in device memory), in the
+// kernel's order (ord: n_slots x nb indices, bins: kAdjJacBins ints, head: two
+// counters, buf: adj_rp_jac_buf_doubles(max_blocks) doubles of reverse-mode
+// states); replays whose sub-step count differs from the taped one are
 // counted in *mismatch.
+constexpr int kAdjJacBins = 8193;
 void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const BubbleArrays &b,
                              double dt_mix, const JacSlot *slots, int n_slots, unsigned *ord,
-                             int *bins, unsigned *head, unsigned *mismatch, int cus, int per_cu);
-// Workgroups of the Jacobian kernel one compute unit holds (its occupancy).
+                             int *bins, unsigned *head, unsigned *mismatch, double *buf,
+                             int max_blocks);
+// Workgroups of the Jacobian kernel one compute unit holds (its occupancy),
+// and the reverse-mode state buffer a grid of `blocks` workgroups needs.
 int adj_rp_jac_per_cu();
+size_t adj_rp_jac_buf_doubles(int blocks);
 // The transpose of one iteration's RP map: LRe (the adjoint of the step's
 // final RP state) only for the last iteration, else null.
 void launch_adj_rp_apply(hipStream_t s, int nb, const double *jac, const double *lRt,

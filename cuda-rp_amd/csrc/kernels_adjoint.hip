@@ -566,17 +566,33 @@ __global__ __launch_bounds__(256) void k_adj_motion(GridParams g, BubbleParams b
 // The RP map of one Picard iteration, (R, Rn, d1R, PG, PL_p, PL_n, PL_m) ->
 // (R', Rn', d1R', PG', R_t), depends only on the forward's taped inputs, so
 // its Jacobian is computed for every Picard iteration of a run of steps in
-// one launch (k_adj_rp_jac, after the items are ordered longest-first by
-// k_jac_rank), and each reverse step applies its transpose to the step's
-// adjoints (k_adj_rp_apply).  Without it every iteration's integration would
-// be a launch of its own, bounded by that iteration's longest chain.
+// one launch (k_adj_rp_jac, after the items are ordered by k_jac_rank), and
+// each reverse step applies its transpose to the step's adjoints
+// (k_adj_rp_apply).  Without it every iteration's integration would be a
+// launch of its own, bounded by that iteration's longest chain.
 
-// Bins of the longest-first order: sub-step counts 0..kJacBins-1 (larger
-// counts share the first bin), descending.
+// Keys of the item order (k_jac_rank): first the items of more than
+// kJacLong sub-steps, longest first (counts from kJacBins on share one key),
+// whose Jacobians k_adj_rp_jac takes in forward mode, seven tangent lanes
+// each, so that a long chain takes one pass; then the rest, in reverse mode
+// on one lane each -- the items of a step's last Picard iteration (five
+// outputs) before the others (one output, R_t), each longest first, so that
+// a wavefront holds items of one kind.  kJacKeys counts the keys; the
+// prefix sum at key kJacBins (the number of long items) is kept after them.
 constexpr int kJacBins = 4096;
+constexpr int kJacLong = 1024;
+constexpr int kJacKeys = 2 * kJacBins;
 constexpr int kJacTile = 4096;  // items per workgroup of k_jac_rank
+// Reverse-mode blocks: every state of an item of at most kJacRevM
+// sub-steps, else every kJacRevM-th and the current block's
+// (kJacRevM * kJacRevC >= kJacLong).
+constexpr int kJacRevM = 32, kJacRevC = kJacLong / kJacRevM;
 
-__device__ inline int jac_bin(int sub) { return kJacBins - 1 - min(max(sub, 0), kJacBins - 1); }
+__device__ inline int jac_key(int sub, bool last) {
+    sub = max(sub, 0);
+    if (sub > kJacLong) return kJacBins - 1 - min(sub, kJacBins - 1);
+    return kJacBins + (last ? 0 : kJacLong + 1) + (kJacLong - sub);
+}
 
 // Two passes over tiles of kJacTile items.  PLACE = false: global counts per
 // bin.  PLACE = true (after k_jac_scan turned the counts into offsets): each
@@ -587,9 +603,9 @@ template <bool PLACE>
 __global__ __launch_bounds__(256) void k_jac_rank(const JacSlot *slots, int nb,
                                                   unsigned n_items, int *bins,
                                                   unsigned *ord) {
-    __shared__ int cnt[kJacBins];
-    __shared__ int base[PLACE ? kJacBins : 1];
-    for (int i = threadIdx.x; i < kJacBins; i += 256) cnt[i] = 0;
+    __shared__ int cnt[kJacKeys];
+    __shared__ int base[PLACE ? kJacKeys : 1];
+    for (int i = threadIdx.x; i < kJacKeys; i += 256) cnt[i] = 0;
     __syncthreads();
     const unsigned t0 = blockIdx.x * (unsigned)kJacTile;
     constexpr int kPer = kJacTile / 256;
@@ -600,20 +616,20 @@ __global__ __launch_bounds__(256) void k_jac_rank(const JacSlot *slots, int nb,
         bin[r] = -1;
         if (q < n_items) {
             const unsigned sl = q / (unsigned)nb, idx = q - sl * (unsigned)nb;
-            bin[r] = jac_bin(slots[sl].sub[idx]);
+            bin[r] = jac_key(slots[sl].sub[idx], slots[sl].last);
             if (!PLACE) atomicAdd(&cnt[bin[r]], 1);
         }
     }
     if constexpr (!PLACE) {
         __syncthreads();
-        for (int i = threadIdx.x; i < kJacBins; i += 256)
+        for (int i = threadIdx.x; i < kJacKeys; i += 256)
             if (cnt[i]) atomicAdd(&bins[i], cnt[i]);
     } else {
 #pragma unroll
         for (int r = 0; r < kPer; r++)
             if (bin[r] >= 0) atomicAdd(&cnt[bin[r]], 1);
         __syncthreads();
-        for (int i = threadIdx.x; i < kJacBins; i += 256) {
+        for (int i = threadIdx.x; i < kJacKeys; i += 256) {
             base[i] = cnt[i] ? atomicAdd(&bins[i], cnt[i]) : 0;
             cnt[i] = 0;
         }
@@ -625,9 +641,10 @@ __global__ __launch_bounds__(256) void k_jac_rank(const JacSlot *slots, int nb,
     }
 }
 
-// Exclusive prefix sum of the kJacBins counts, in place (one workgroup).
+// Exclusive prefix sum of the kJacKeys counts, in place (one workgroup);
+// bins[kJacKeys] = the number of long items.
 __global__ __launch_bounds__(1024) void k_jac_scan(int *bins) {
-    constexpr int kPer = kJacBins / 1024;
+    constexpr int kPer = kJacKeys / 1024;
     __shared__ int part[1024];
     int v[kPer], sum = 0;
 #pragma unroll
@@ -647,6 +664,7 @@ __global__ __launch_bounds__(1024) void k_jac_scan(int *bins) {
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
         bins[threadIdx.x * kPer + r] = run;
+        if (threadIdx.x * kPer + r == kJacBins) bins[kJacKeys] = run;
         run += v[r];
     }
 }
@@ -676,33 +694,77 @@ __device__ inline void jac_item(const BubbleParams &bp, const BubbleArrays &b, d
     if (d == 0 && S.sub[idx] != L.counter) atomicAdd(mismatch, 1u);
 }
 
-// Jacobian columns of the RP map, seven lanes per item (one Picard
-// iteration of one bubble), nine items per wavefront (lane 63 idles): lane
-// d runs the sub-step in (value, tangent) arithmetic with the tangent seeded
-// on input d (0..3 start state R, Rn, d1R, PG; 4 PL_p; 5 PL_n; 6 PL_m).  The
-// value part repeats the forward's operations exactly, so each lane takes
-// the forward's sub-steps; lane 0 counts a difference from the taped count
-// in *mismatch.  Output: jac[c * nb + idx] for c = 7 o + d, o = 0 (R_t) and,
-// in the step's last iteration, o = 1..4 (R', Rn', d1R', PG').  Persistent:
-// each wavefront takes the next nine items of ord until none is left.
+// One item of k_adj_rp_jac in reverse mode, on one lane (rp_jac_rev).
+template <int NOUT>
+__device__ inline void jac_item_rev(const BubbleParams &bp, const BubbleArrays &b, double dt_mix,
+                                    const JacSlot &S, unsigned idx, const RPFastShared &fs,
+                                    const RPRevBuf &B, unsigned *mismatch) {
+    const RPFastConst fk = rp_fast_const(b.R0[idx], b.PG0[idx], b.calpha[idx], bp.T0, fs);
+    RPLaneF<double> L0;
+    rp_begin_g(L0, S.rp[0][idx], S.rp[1][idx], S.rp[2][idx], S.rp[3][idx], S.rp[4][idx],
+               S.rp[5][idx], S.PLm[idx], S.PLn[idx], S.PLp[idx], dt_mix, bp.PL0);
+    if (!rp_jac_rev<NOUT>(L0, S.sub[idx], S.last ? 5 : 1, bp.max_substeps, dt_mix, fk, fs, B,
+                          S.jac + idx, (size_t)b.n))
+        atomicAdd(mismatch, 1u);
+}
+
+// Jacobian columns of the RP map: jac[c * nb + idx] for c = 7 o + d, o = 0
+// (R_t) and, in the step's last iteration, o = 1..4 (R', Rn', d1R', PG'),
+// d = 0..6 (inputs R, Rn, d1R, PG, PL_p, PL_n, PL_m).  Persistent, over the
+// order of k_jac_rank: first the long items (more than kJacLong sub-steps),
+// seven lanes each, nine per wavefront (lane 63 idles): lane d runs the
+// sub-steps in (value, tangent) arithmetic with the tangent seeded on input
+// d, so a long chain takes one pass; its value part repeats the forward's
+// operations exactly, so each lane takes the forward's sub-steps, and lane 0
+// counts a difference from the taped count in *mismatch.  Then the other
+// items, 64 per wavefront, in reverse mode (rp_jac_rev): one forward pass
+// keeping the states, one reverse pass with all of the item's output
+// adjoints (the wavefront's items are of one kind but at the boundary, where
+// it carries five).  Per item and sub-step the reverse mode issues about a
+// fifth of the forward mode's instructions.  buf holds kJacRevC + kJacRevM
+// seven-double records per lane of the grid.
 constexpr int kJacPerWave = 9;
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_adj_rp_jac(BubbleParams bp, BubbleArrays b, double dt_mix,
-                                                     const JacSlot *slots, const unsigned *ord,
-                                                     unsigned n_items, unsigned *head,
-                                                     unsigned *mismatch) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_rp_jac(
+    BubbleParams bp, BubbleArrays b, double dt_mix, const JacSlot *slots, const unsigned *ord,
+    unsigned n_items, const int *bins, unsigned *head, unsigned *mismatch, double *buf) {
     __shared__ double vtab[kRpVSize];  // the U and Y tables
     rp_vtab_stage(bp.rp_shared, vtab);
     __syncthreads();
     const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);
+    const unsigned n_long = (unsigned)bins[kJacKeys];
     const int lane = threadIdx.x & 63, slot = lane / 7, d = lane - 7 * slot;
     for (;;) {
         unsigned first = 0;
-        if (lane == 0) first = atomicAdd(head, (unsigned)kJacPerWave);
+        if (lane == 0) first = atomicAdd(&head[0], (unsigned)kJacPerWave);
         first = __shfl(first, 0);
-        if (first >= n_items) break;
+        if (first >= n_long) break;
         const unsigned q = first + slot;
-        if (slot < kJacPerWave && q < n_items)
+        if (slot < kJacPerWave && q < n_long)
             jac_item(bp, b, dt_mix, slots, ord[q], d, fs, mismatch);
+    }
+    const size_t nl = (size_t)gridDim.x * blockDim.x;
+    const size_t gl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const RPRevBuf B{buf + gl, buf + (size_t)kJacRevC * 7 * nl + gl, nl, kJacRevM, kJacRevC};
+    const unsigned nb = (unsigned)b.n;
+    for (;;) {
+        unsigned first = 0;
+        if (lane == 0) first = atomicAdd(&head[1], 64u);
+        first = __shfl(first, 0);
+        if (first >= n_items - n_long) break;
+        const unsigned q = n_long + first + lane;
+        const bool has = q < n_items;
+        unsigned sl = 0, idx = 0;
+        if (has) {
+            const unsigned item = ord[q];
+            sl = item / nb;
+            idx = item - sl * nb;
+        }
+        const bool last = has && slots[sl].last;
+        if (__ballot(last) != 0ull) {
+            if (has) jac_item_rev<5>(bp, b, dt_mix, slots[sl], idx, fs, B, mismatch);
+        } else if (has) {
+            jac_item_rev<1>(bp, b, dt_mix, slots[sl], idx, fs, B, mismatch);
+        }
     }
 }
 
@@ -861,6 +923,10 @@ void launch_adj_motion(hipStream_t s, const GridParams &g, const BubbleParams &b
                        vx, vy, Lvx, Lvy, Lx, Ly);
 }
 
+size_t adj_rp_jac_buf_doubles(int blocks) {
+    return (size_t)blocks * 256 * (kJacRevC + kJacRevM) * 7;
+}
+
 int adj_rp_jac_per_cu() {
     int nblk = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k_adj_rp_jac, 256, 0) != hipSuccess ||
@@ -871,21 +937,22 @@ int adj_rp_jac_per_cu() {
 
 void launch_adj_rp_jacobians(hipStream_t s, const BubbleParams &bp, const BubbleArrays &b,
                              double dt_mix, const JacSlot *slots, int n_slots, unsigned *ord,
-                             int *bins, unsigned *head, unsigned *mismatch, int cus, int per_cu) {
+                             int *bins, unsigned *head, unsigned *mismatch, double *buf,
+                             int max_blocks) {
     if (b.n <= 0 || n_slots <= 0) return;
     const unsigned n_items = (unsigned)((long)n_slots * b.n);
     const unsigned tiles = (n_items + kJacTile - 1) / kJacTile;
-    hipLaunchKernelGGL(k_fill_i, dim3((kJacBins + 255) / 256), dim3(256), 0, s, bins, kJacBins);
-    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(256), 0, s, (int *)head, 1);
+    hipLaunchKernelGGL(k_fill_i, dim3((kJacKeys + 1 + 255) / 256), dim3(256), 0, s, bins,
+                       kJacKeys + 1);
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(256), 0, s, (int *)head, 2);
     hipLaunchKernelGGL(k_jac_rank<false>, dim3(tiles), dim3(256), 0, s, slots, b.n, n_items, bins,
                        ord);
     hipLaunchKernelGGL(k_jac_scan, dim3(1), dim3(1024), 0, s, bins);
     hipLaunchKernelGGL(k_jac_rank<true>, dim3(tiles), dim3(256), 0, s, slots, b.n, n_items, bins,
                        ord);
-    const long waves = ((long)n_items + kJacPerWave - 1) / kJacPerWave;
-    const long blocks = std::min<long>((long)per_cu * cus, (waves + 3) / 4);
+    const long blocks = std::max<long>(1, std::min<long>(max_blocks, ((long)n_items + 255) / 256));
     hipLaunchKernelGGL(k_adj_rp_jac, dim3((unsigned)blocks), dim3(256), 0, s, bp, b, dt_mix, slots,
-                       ord, n_items, head, mismatch);
+                       ord, n_items, bins, head, mismatch, buf);
 }
 
 void launch_adj_rp_apply(hipStream_t s, int nb, const double *jac, const double *lRt,

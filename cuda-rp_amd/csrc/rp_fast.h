@@ -287,10 +287,58 @@ __device__ inline void rp_begin_g(RPLaneF<T> &L, T Rp, T Rn, T d1R, T PG, double
     L.counter = 0;
 }
 
+// The step-size control of solveRayleighPlesset (bubble_CUDA_kernel.cuh:
+// 235-243): the previous step times 1.01, capped by dt0, by 1 % of R over
+// |dR/dt| and by 10 % of R over sqrt|R d2R/dt2|.
+__device__ inline double rp_dtp(double dtm, double Rn, double d1R, double Rnd2R,
+                                const RPFastShared &s) {
+    double dtp = fmin(1.01 * dtm, s.dt0);
+    const double ad = fabs(d1R), aa = fabs(Rnd2R);
+    if (ad > kDblEps) dtp = fmin(dtp, (0.01 * Rn) * drcp(ad));
+    if (aa > kDblEps) dtp = fmin(dtp, (0.1 * Rn) * drsqrt(aa));
+    return dtp;
+}
+
+// solveOmegaN at (PGn, Rn) (bubble_CUDA_kernel.cuh:263-289) and solveLp as
+// solvePG uses it (Lp / |Lp|^2), with iRn = 1/Rn: *Ln = Lp / |Lp|^2 and
+// *iom = 1/omega_N.  PGn cancels from the iteration: with
+// coef1 = PG0 / (PGn Rx^3) and coef2 = PGn rho,
+// Upsilon.re coef1 coef2 = (Upsilon.re / 3 gam) A and
+// Upsilon.im coef1 PGn / 4 = (-Upsilon.im / 3 gam) B with A = A0 / Rn^3,
+// B = B0 / Rn^3 (rp_fast_const), so both depend on Rn alone.
+template <typename T>
+__device__ inline void omega_lp_g(T Rn, T iRn, const RPFastConst &k, const RPFastShared &s,
+                                  cx<T> *Ln, T *iom_out) {
+    const T iRn2 = iRn * iRn;
+    const T iRn3 = iRn2 * iRn;
+    const T A = k.A0 * iRn3, B = k.B0 * iRn3;
+    const T coef3 = 4.0 * iRn2;
+    const T value1 = s.m2sig_rho * iRn;
+    const T scale = iRn * s.irho;
+    const T iscale = Rn * s.rho;
+    T eta = A + fma_(T(-s.mu2), coef3, value1);
+    T om, iom;
+    omega_g(eta, scale, iscale, &om, &iom);
+    const T ca = k.calpha * iRn;
+    for (int it = 0; it < 3; it++) {
+        const T x = ca * om;
+        const T isa = rsqrt_(x);
+        const cx<T> ups = upsilon_g(x * isa, isa, s.k3, s.vtab);
+        const T mu_eff = fma_(ups.im * B, iom, T(s.mu));  // mu + ui PG / (4 omega)
+        eta = fma_(ups.re, A, fma_(-(mu_eff * mu_eff), coef3, value1));
+        omega_g(eta, scale, iscale, &om, &iom);
+    }
+    const T x = ca * om;
+    const T isa = rsqrt_(x);
+    *Ln = lp_inv_g(x * isa, isa, iRn, s.vtab);
+    *iom_out = iom;
+}
+
 // One pass of the sub-step loop body (bubble_CUDA_kernel.cuh:508-538).
 // FORCED: the step size is forced_dtp instead of the step-size control's
 // (the replay of a recorded schedule; the rest of the sub-step is unchanged).
-template <typename T, bool FORCED = false>
+// HEAT: accumulate the heat and viscous dissipation of Q_B (T = double).
+template <typename T, bool FORCED = false, bool HEAT = true>
 __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const RPFastShared &s,
                                     double forced_dtp = 0.0) {
     // solveRayleighPlesset (bubble_CUDA_kernel.cuh:227-260)
@@ -304,10 +352,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     const T sv = iRn * fma_(T(s.four_mu), d1R, T(s.two_sig));  // (2 sig + 4 mu dR) / R
     const T Rnd2R = fma_(-1.5 * d1R, d1R, (PGn - PL - sv) * s.irho);
     const double dtm = L.dtL;
-    double dtp = fmin(1.01 * dtm, s.dt0);
-    const double ad = fabs(val(d1R)), aa = fabs(val(Rnd2R));
-    if (ad > kDblEps) dtp = fmin(dtp, (0.01 * val(Rn)) * drcp(ad));
-    if (aa > kDblEps) dtp = fmin(dtp, (0.1 * val(Rn)) * drsqrt(aa));
+    double dtp = rp_dtp(dtm, val(Rn), val(d1R), val(Rnd2R), s);
     if constexpr (FORCED) dtp = forced_dtp;
     const double idtm = drcp(dtm);
     const double r = dtp * idtm;
@@ -331,33 +376,9 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     L.Rp = Rp;
     L.Rn = Rn;
     L.d1R = d1R;
-    // solveOmegaN at (PGn, Rn) (bubble_CUDA_kernel.cuh:263-289).  PGn cancels
-    // from the iteration: with coef1 = PG0 / (PGn Rx^3) and coef2 = PGn rho,
-    // Upsilon.re coef1 coef2 = (Upsilon.re / 3 gam) A and
-    // Upsilon.im coef1 PGn / 4 = (-Upsilon.im / 3 gam) B with A = A0 / Rn^3,
-    // B = B0 / Rn^3 (rp_fast_const), so the iteration depends on Rn alone.
-    const T iRn2 = iRn * iRn;
-    const T iRn3 = iRn2 * iRn;
-    const T A = k.A0 * iRn3, B = k.B0 * iRn3;
-    const T coef3 = 4.0 * iRn2;
-    const T value1 = s.m2sig_rho * iRn;
-    const T scale = iRn * s.irho;
-    const T iscale = Rn * s.rho;
-    T eta = A + fma_(T(-s.mu2), coef3, value1);
-    T om, iom;
-    omega_g(eta, scale, iscale, &om, &iom);
-    const T ca = k.calpha * iRn;
-    for (int it = 0; it < 3; it++) {
-        const T x = ca * om;
-        const T isa = rsqrt_(x);
-        const cx<T> ups = upsilon_g(x * isa, isa, s.k3, s.vtab);
-        const T mu_eff = fma_(ups.im * B, iom, T(s.mu));  // mu + ui PG / (4 omega)
-        eta = fma_(ups.re, A, fma_(-(mu_eff * mu_eff), coef3, value1));
-        omega_g(eta, scale, iscale, &om, &iom);
-    }
-    const T x = ca * om;
-    const T isa = rsqrt_(x);
-    const cx<T> Ln = lp_inv_g(x * isa, isa, iRn, s.vtab);  // Lp / |Lp|^2
+    cx<T> Ln;  // Lp / |Lp|^2
+    T iom;
+    omega_lp_g(Rn, iRn, k, s, &Ln, &iom);
     // solvePG (bubble_CUDA_kernel.cuh:322-340)
     const T Rbar = 0.5 * (Rp + Rn);
     const T iRbar = rcp_(Rbar);
@@ -371,7 +392,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     const T c2 = fma_(-q, Rnx * Rnx * Rnx, T(1.0) - c3);
     const T PGp = fma_(c2, PGn, (c0 * Lnr) * k.PG0) * rcp_(c1);
     L.PG = PGp;
-    if constexpr (std::is_same<T, double>::value) {
+    if constexpr (HEAT && std::is_same<T, double>::value) {
         // heat and viscous dissipation (Q_B): dTdr dt with the 1/dt of its
         // second term cancelled against the dt factor
         // Lp.re / |Lp|^2 = Lnr, Lp.im / |Lp|^2 = Lni
@@ -382,6 +403,211 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
         L.SumVis = fma(s.sixteen_pi_mu * Rp * dtL, d1R * d1R, L.SumVis);
     }
     L.counter++;
+}
+
+// ---- reverse mode: adjoints of the RP map of one Picard iteration ----------
+// The lane state before a sub-step, as the reverse pass restores it.
+struct RPPre {
+    double Rp, Rn, d1R, PG, dtL, remain, time;
+};
+
+// Reverse of rp_substep_g<double> at the frozen schedule (the step size, the
+// previous one and the time held at the forward's values), for NOUT adjoint
+// vectors at once.  On entry l[.][o] holds the adjoints of the sub-step's
+// outputs (Rp, Rn, d1R, PG) and lRt[o] that of R_t, used when the sub-step
+// is the integration's final one (`final`); on exit l[.][o] holds the
+// adjoints of its inputs, and lPC[.][o] has the adjoints of the pressure
+// coefficients PC0, PC1, PC2 added.  The Omega iteration and Lp depend on Rn
+// alone (omega_lp_g), so their derivative is one tangent along Rn;
+// solveRayleighPlesset and solvePG are reversed statement by statement.
+template <int NOUT>
+__device__ inline void rp_substep_rev(const RPPre &P, bool final, const double (&lRt)[NOUT],
+                                      double PC0, double PC1, double PC2, const RPFastConst &k,
+                                      const RPFastShared &s, double (&l)[4][NOUT],
+                                      double (&lPC)[3][NOUT]) {
+    // the sub-step's values, as rp_substep_g<double> computes them
+    const double PGn = P.PG, t = P.time, dtm = P.dtL;
+    const double PL = fma(PC2, t * t, fma(PC1, t, PC0));
+    const double Rm = P.Rn, Rn = P.Rp, d1R = P.d1R;
+    const double iRn = drcp(Rn);
+    const double sva = fma(s.four_mu, d1R, s.two_sig);
+    const double Rnd2R = fma(-1.5 * d1R, d1R, (PGn - PL - iRn * sva) * s.irho);
+    const double dtp = rp_dtp(dtm, Rn, d1R, Rnd2R, s);
+    const double idtm = drcp(dtm);
+    const double r = dtp * idtm, h1 = 0.5 * (dtp + dtm);
+    const double acc = (dtp * Rnd2R) * iRn;
+    const double Rp = fma(acc, h1, fma(1.0 + r, Rn, -r * Rm));
+    const double dm2 = dtm * dtm, dp2 = dtp * dtp;
+    const double iden = drcp(dtm * dtp * (dtm + dtp));
+    cx<dd> Ld;
+    dd iomd;
+    const dd Rnd(Rn, 1.0);
+    omega_lp_g<dd>(Rnd, rcp_(Rnd), k, s, &Ld, &iomd);
+    const double Lnr = Ld.re.v, Lni = Ld.im.v, iom = iomd.v;
+    const double Rbar = 0.5 * (Rp + Rn), iRbar = drcp(Rbar);
+    const double Rx = Rbar * k.iR0, Rpx = Rp * k.iR0, Rnx = Rn * k.iR0;
+    const double Rx2 = Rx * Rx, Rpx2 = Rpx * Rpx, Rnx2 = Rnx * Rnx;
+    const double cdt = (s.C0 * iRbar) * k.iPG0, c0 = dtp * cdt;
+    const double hg = 1.5 * s.gam, hx3 = 0.5 * (Rx2 * Rx);
+    const double c3 = fma(c0 * Lnr, hx3, (hg * (Rp - Rn)) * iRbar);
+    const double q = (cdt * Lni) * iom;
+    const double c1 = fma(-q, Rpx2 * Rpx, 1.0 + c3), c2 = fma(-q, Rnx2 * Rnx, 1.0 - c3);
+    const double ic1 = drcp(c1);
+    const double PGp = fma(c2, PGn, (c0 * Lnr) * k.PG0) * ic1;
+    // R_t of the final sub-step: at the sync time, or Rp when the sub-step
+    // cap ended the integration
+    const bool sync = final && dtp >= P.remain;
+    const double d = P.remain, r2 = d * idtm, h2 = 0.5 * (d + dtm);
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) {
+        // solvePG: PGp = (c2 PGn + c0 Lnr PG0) / c1
+        const double lN = l[3][o] * ic1, lc1 = -l[3][o] * PGp * ic1, lc2 = lN * PGn;
+        double lPGn = lN * c2;
+        double lc0 = lN * Lnr * k.PG0, lLnr = lN * c0 * k.PG0;
+        const double lc3 = lc1 - lc2;
+        const double lq = -(lc1 * (Rpx2 * Rpx) + lc2 * (Rnx2 * Rnx));
+        const double lRpx = -3.0 * lc1 * q * Rpx2, lRnx = -3.0 * lc2 * q * Rnx2;
+        double lcdt = lq * Lni * iom;
+        const double lLni = lq * cdt * iom, liom = lq * cdt * Lni;
+        lc0 += lc3 * Lnr * hx3;
+        lLnr += lc3 * c0 * hx3;
+        const double lRx = 1.5 * lc3 * c0 * Lnr * Rx2;
+        double liRbar = lc3 * hg * (Rp - Rn);
+        double lRp = lc3 * hg * iRbar, lRn = -lc3 * hg * iRbar;
+        lcdt += lc0 * dtp;
+        liRbar += lcdt * s.C0 * k.iPG0;
+        const double lRbar = lRx * k.iR0 - liRbar * iRbar * iRbar;
+        lRp += lRpx * k.iR0 + 0.5 * lRbar;
+        lRn += lRnx * k.iR0 + 0.5 * lRbar;
+        // the Omega iteration and Lp
+        lRn += lLnr * Ld.re.d + lLni * Ld.im.d + liom * iomd.d;
+        // solveRayleighPlesset: d1R' = (dm2 Rp + (dp2 - dm2) Rn - dp2 Rm) iden
+        // + acc, Rp = acc h1 + (1 + r) Rn - r Rm, acc = dtp Rnd2R / Rn
+        const double lS = l[2][o] * iden;
+        double lacc = l[2][o];
+        lRp += l[0][o] + lS * dm2;
+        lRn += l[1][o] + lS * (dp2 - dm2);
+        double lRm = -lS * dp2;
+        double lRnd2R = 0.0, liRn = 0.0;
+        if (sync) {
+            lRnd2R += lRt[o] * d * iRn * h2;
+            liRn += lRt[o] * d * Rnd2R * h2;
+            lRn += lRt[o] * (1.0 + r2);
+            lRm -= lRt[o] * r2;
+        } else if (final) {
+            lRp += lRt[o];
+        }
+        lacc += lRp * h1;
+        lRn += lRp * (1.0 + r);
+        lRm -= lRp * r;
+        lRnd2R += lacc * dtp * iRn;
+        liRn += lacc * dtp * Rnd2R;
+        // Rnd2R = -1.5 d1R^2 + (PGn - PL - sva / Rn) / rho
+        const double lQ = lRnd2R * s.irho;
+        const double ld1R = -3.0 * lRnd2R * d1R - lQ * iRn * s.four_mu;
+        lPGn += lQ;
+        liRn -= lQ * sva;
+        lRn -= liRn * iRn * iRn;
+        lPC[0][o] -= lQ;
+        lPC[1][o] -= lQ * t;
+        lPC[2][o] -= lQ * (t * t);
+        l[0][o] = lRn;  // the input Rp is this sub-step's Rn
+        l[1][o] = lRm;  // the input Rn is its Rm
+        l[2][o] = ld1R;
+        l[3][o] = lPGn;
+    }
+}
+
+// Per-lane storage of the reverse pass: `ck` records every m-th lane state
+// (c of them), `blk` the m states of the block being reversed; each record is
+// seven doubles, record r field f at base[(r * 7 + f) * stride].
+struct RPRevBuf {
+    double *ck, *blk;
+    size_t stride;
+    int m, c;
+};
+__device__ inline void rev_put(double *p, size_t st, const RPLaneF<double> &L) {
+    p[0] = L.Rp;
+    p[st] = L.Rn;
+    p[2 * st] = L.d1R;
+    p[3 * st] = L.PG;
+    p[4 * st] = L.dtL;
+    p[5 * st] = L.remain;
+    p[6 * st] = L.time;
+}
+__device__ inline RPPre rev_get(const double *p, size_t st) {
+    return RPPre{p[0], p[st], p[2 * st], p[3 * st], p[4 * st], p[5 * st], p[6 * st]};
+}
+
+// Jacobian rows of the RP map of one Picard iteration on one lane:
+// J[(7 o + d) * nb] = dOut_o / dIn_d for the outputs o < nout <= NOUT (R_t,
+// then R', Rn', d1R', PG') and the inputs d (R, Rn, d1R, PG, PL_p, PL_n,
+// PL_m).  The forward pass repeats the forward's sub-steps (their count must
+// be K, the forward's; returns false otherwise) and keeps every lane state,
+// or every m-th when K > m, whose blocks are integrated again from there
+// during the reverse pass, which carries all NOUT adjoint vectors at once.
+template <int NOUT>
+__device__ inline bool rp_jac_rev(const RPLaneF<double> &L0, int K, int nout, int max_substeps,
+                                  double dt_mix, const RPFastConst &k, const RPFastShared &s,
+                                  const RPRevBuf &B, double *J, size_t nb) {
+    const bool two = K > B.m;
+    const int cap = two ? B.c * B.m : B.m;
+    RPLaneF<double> L = L0;
+    while (L.remain > 0.0 && L.counter < max_substeps && L.counter < cap) {
+        const int kk = L.counter;
+        if (!two)
+            rev_put(B.blk + (size_t)kk * 7 * B.stride, B.stride, L);
+        else if (kk % B.m == 0)
+            rev_put(B.ck + (size_t)(kk / B.m) * 7 * B.stride, B.stride, L);
+        rp_substep_g<double, false, false>(L, k, s);
+    }
+    if (L.counter != K || (L.remain > 0.0 && L.counter < max_substeps)) return false;
+    double l[4][NOUT], lPC[3][NOUT], lRt[NOUT];
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) l[j][o] = o == j + 1 ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) lPC[j][o] = 0.0;
+        lRt[o] = o == 0 ? 1.0 : 0.0;
+    }
+    for (int b = two ? (K - 1) / B.m : 0; b >= 0; b--) {
+        const int k0 = two ? b * B.m : 0, k1 = two ? min(K, k0 + B.m) : K;
+        if (two) {  // the block's states again, from its checkpoint
+            RPLaneF<double> R = L0;
+            const RPPre c = rev_get(B.ck + (size_t)b * 7 * B.stride, B.stride);
+            R.Rp = c.Rp;
+            R.Rn = c.Rn;
+            R.d1R = c.d1R;
+            R.PG = c.PG;
+            R.dtL = c.dtL;
+            R.remain = c.remain;
+            R.time = c.time;
+            R.counter = k0;
+            for (int kk = k0; kk < k1; kk++) {
+                rev_put(B.blk + (size_t)(kk - k0) * 7 * B.stride, B.stride, R);
+                rp_substep_g<double, false, false>(R, k, s);
+            }
+        }
+        for (int kk = k1 - 1; kk >= k0; kk--)
+            rp_substep_rev<NOUT>(rev_get(B.blk + (size_t)(kk - k0) * 7 * B.stride, B.stride),
+                                 kk == K - 1, lRt, L0.PC0, L0.PC1, L0.PC2, k, s, l, lPC);
+    }
+    // PC0 = PL_n + PL0, PC1 = (PL_p - PL_m) / (2 dt), PC2 = (PL_p + PL_m - 2 PL_n) / (2 dt^2)
+    const double idt = 1.0 / dt_mix;
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) {
+        if (o >= nout) break;
+        double *Jo = J + (size_t)(7 * o) * nb;
+        Jo[0] = l[0][o];
+        Jo[nb] = l[1][o];
+        Jo[2 * nb] = l[2][o];
+        Jo[3 * nb] = l[3][o];
+        Jo[4 * nb] = 0.5 * idt * (lPC[1][o] + lPC[2][o] * idt);
+        Jo[5 * nb] = lPC[0][o] - lPC[2][o] * idt * idt;
+        Jo[6 * nb] = 0.5 * idt * (lPC[2][o] * idt - lPC[1][o]);
+    }
+    return true;
 }
 
 // Exit of BubbleRadiusKernel (bubble_CUDA_kernel.cuh:542-550).
