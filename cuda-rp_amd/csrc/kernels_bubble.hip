@@ -1009,20 +1009,25 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int *sub, const int 
 // longest RP chains belong to bubbles that are already collapsing fast when
 // the step starts, and they are other bubbles every step, so the previous
 // step's counts do not find them.  A bubble shrinking faster than
-// kCollapseSpeed gets the key 10 (-dR/dt) R / R0, an estimate of its
-// sub-steps (on the 4096^2 clustered workload the median count is about ten
-// times that score up to 200 m/s, and ordered by it the 16 longest chains of
-// a step sit within its first 300 queue entries and the 4096 longest within
+// kCollapseSpeed gets the key 10 (-dR/dt) (R / R0) sqrt(kKeyRadius / R0), an
+// estimate of its sub-steps (on the 4096^2 clustered workload the median
+// count is about ten times (-dR/dt) R / R0 up to 200 m/s; ordered by the key,
+// the 16 longest chains of a step sit within its first 150 queue entries,
+// 94 % of the 64 longest within the first 256, and the 4096 longest within
 // the first 35 000 of 4.2 million); every other bubble keeps its previous
 // count, capped below the extreme class.
 constexpr double kCollapseSpeed = 25.0;  // m/s
+constexpr double kKeyRadius = 3.0e-6;    // m
 
 __global__ __launch_bounds__(256) void k_first_keys(BubbleArrays b, int cur, int ext) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= b.n) return;
     const double v = b.d1R[cur][idx];
     int k = min(b.sub[idx], ext - 1);
-    if (v < -kCollapseSpeed) k = (int)fmin(-10.0 * v * (b.R[cur][idx] / b.R0[idx]), 1.0e6);
+    if (v < -kCollapseSpeed) {
+        const double R0 = b.R0[idx];
+        k = (int)fmin(-10.0 * v * (b.R[cur][idx] / R0) * sqrt(kKeyRadius / R0), 1.0e6);
+    }
     b.key[idx] = k;
 }
 
