@@ -1014,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int *sub, const int 
 // count is about ten times (-dR/dt) R / R0 up to 200 m/s; ordered by the key,
 // the 16 longest chains of a step sit within its first 150 queue entries,
 // 94 % of the 64 longest within the first 256, and the 4096 longest within
-// the first 35 000 of 4.2 million); every other bubble keeps its previous
+// the first 26 000 of 4.2 million); every other bubble keeps its previous
 // count, capped below the extreme class.
 constexpr double kCollapseSpeed = 25.0;  // m/s
 constexpr double kKeyRadius = 3.0e-6;    // m
