@@ -289,13 +289,16 @@ __device__ inline void rp_begin_g(RPLaneF<T> &L, T Rp, T Rn, T d1R, T PG, double
 
 // The step-size control of solveRayleighPlesset (bubble_CUDA_kernel.cuh:
 // 235-243): the previous step times 1.01, capped by dt0, by 1 % of R over
-// |dR/dt| and by 10 % of R over sqrt|R d2R/dt2|.
+// |dR/dt| and by 10 % of R over sqrt|R d2R/dt2| (each limit only where its
+// denominator exceeds DBL_EPSILON).  The two limits share one reciprocal
+// square root: min(0.01 R / |dR/dt|, 0.1 R / sqrt|R d2R/dt2|) =
+// R / sqrt(max(1e4 (dR/dt)^2, 100 |R d2R/dt2|)).
 __device__ inline double rp_dtp(double dtm, double Rn, double d1R, double Rnd2R,
                                 const RPFastShared &s) {
     double dtp = fmin(1.01 * dtm, s.dt0);
     const double ad = fabs(d1R), aa = fabs(Rnd2R);
-    if (ad > kDblEps) dtp = fmin(dtp, (0.01 * Rn) * drcp(ad));
-    if (aa > kDblEps) dtp = fmin(dtp, (0.1 * Rn) * drsqrt(aa));
+    const double m = fmax(ad > kDblEps ? 1.0e4 * (ad * ad) : 0.0, aa > kDblEps ? 100.0 * aa : 0.0);
+    if (m > 0.0) dtp = fmin(dtp, Rn * drsqrt(m));
     return dtp;
 }
 
