@@ -861,38 +861,33 @@ __global__ __launch_bounds__(256) void k_part_count(const int *sub, int n, int e
     }
 }
 
-__global__ __launch_bounds__(1024) void k_part_scan(int *cnt, int nblk) {
+// Exclusive prefix sums of the per-block class counts, in place, one
+// workgroup per class (cntE, then cntH = cntE + nblk + 1): each thread sums a
+// run of consecutive blocks, the workgroup scans the runs, each thread
+// writes its run's prefixes; cnt[nblk] = the class total.
+__global__ __launch_bounds__(1024) void k_part_scan(int *cnt0, int nblk) {
+    int *cnt = cnt0 + (size_t)blockIdx.x * (nblk + 1);
     __shared__ int wsum[16];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int base = 0; base < nblk; base += 1024) {
-        const int k = base + threadIdx.x;
-        const int v = k < nblk ? cnt[k] : 0;
-        int x = v;  // inclusive wave scan
-        for (int off = 1; off < 64; off <<= 1) {
-            const int y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int acc = 0;
-            for (int q = 0; q < 16; q++) {
-                const int t = wsum[q];
-                wsum[q] = acc;
-                acc += t;
-            }
-        }
-        __syncthreads();
-        const int excl = carry + wsum[w] + x - v;
-        if (k < nblk) cnt[k] = excl;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = excl + v;
-        __syncthreads();
+    const int per = (nblk + 1023) / 1024;
+    const int b0 = min(nblk, (int)threadIdx.x * per), b1 = min(nblk, b0 + per);
+    int sum = 0;
+    for (int k = b0; k < b1; k++) sum += cnt[k];
+    int x = sum;  // inclusive wave scan of the runs
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    if (threadIdx.x == 0) cnt[nblk] = carry;
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int run = x - sum;
+    for (int q = 0; q < w; q++) run += wsum[q];
+    for (int k = b0; k < b1; k++) {
+        const int v = cnt[k];
+        cnt[k] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) cnt[nblk] = run;
 }
 
 __device__ inline int block_rank(bool f, int *wcnt, int *total) {
@@ -1043,8 +1038,7 @@ void launch_order_partition(hipStream_t s, const BubbleArrays &b, const int *key
     int *cntE = b.part, *cntH = b.part + (nblk + 1);
     hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, s, key, b.n, ext, threshold,
                        cntE, cntH, b.sort_hist);
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntE, nblk);
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, cntH, nblk);
+    hipLaunchKernelGGL(k_part_scan, dim3(2), dim3(1024), 0, s, cntE, nblk);
     hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(256), 0, s, key, b.n, ext, threshold,
                        cntE, cntH, nblk, b.order, b.order_tmp);
     const int sblk = std::min(1024, nblk);
