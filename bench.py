@@ -111,20 +111,23 @@ def summarize(st, wall, ph, n, nb, prec):
             "rp_launch_ms": ph["bubble_rp"] / max(launches, 1),
             "rp_max_substeps": max(s["rp_max_substeps"] for s in st),
             "rp_lean_launches": sum(s["rp_lean_launches"] for s in st),
+            "rp_kept_fraction": sum(s["rp_reused"] for s in st) / max(launches * nb, 1),
             "phase_ms_per_step": {k: v / steps for k, v in ph.items()},
             "device_ms_per_step": sum(ph.values()) / steps,
             "step_bytes": step_bytes / steps}
 
 
 def rp_roofline(sm, nb, traffic):
-    """The dominant kernel, k_bubble_persistent (gather + RP + scatter of every
-    bubble, one launch per Picard iteration): algorithmic FP64 operations per
-    launch over its mean launch time from HIP events on the library's stream."""
+    """The dominant kernel, k_bubble_persistent (RP + scatter of every bubble,
+    one launch per Picard iteration, after k_queue_keys gathered the
+    pressures): algorithmic FP64 operations of the sub-steps it integrated
+    (results kept from the previous iteration count none) per launch over its
+    mean launch time from HIP events on the library's stream."""
     flops = (RP_FLOPS_PER_SUBSTEP * sm["substeps"] + BUBBLE_FLOPS_PER_CALL * nb * sm["launches"]) \
         / max(sm["launches"], 1)
     achieved = flops / (sm["rp_launch_ms"] * 1e-3) / 1e12
-    return {"kernel": "k_bubble_persistent (pressure gather + Rayleigh-Plesset + void-fraction "
-                      "scatter, one launch per Picard iteration)",
+    return {"kernel": "k_bubble_persistent (Rayleigh-Plesset + void-fraction scatter, one "
+                      "launch per Picard iteration)",
             "bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "avg_launch_ms": sm["rp_launch_ms"], "flops_per_launch": flops,

@@ -78,7 +78,7 @@ class StepStats(C.Structure):
         ("nstep", C.c_int64), ("time", C.c_double), ("resimax0", C.c_double),
         ("resimax", C.c_double), ("n_pic", C.c_int32), ("rp_max_substeps", C.c_int32),
         ("rp_mean_substeps", C.c_double), ("status", C.c_int32), ("rp_lean_launches", C.c_int32),
-        ("focal_x", C.c_double), ("focal_y", C.c_double),
+        ("focal_x", C.c_double), ("focal_y", C.c_double), ("rp_reused", C.c_int64),
     ]
 
 

@@ -939,15 +939,14 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                 const StepRecord *tin = ctl ? ctl->tape_in : nullptr;
                 const bool replay = h->sched_mode == 2;
                 if (!tin && !replay) {
-                    // the first iteration's queue from the start state (the
-                    // previous step's counts do not find this step's longest
-                    // chains), the later ones from the previous iteration's counts
-                    const int *key = h->b.sub;
-                    if (loop == 1) {
-                        launch_first_keys(s, h->b, h->rp_cur, h->bp.extreme_threshold);
-                        key = h->b.key;
-                    }
-                    launch_order_partition(s, h->b, key, h->bp.heavy_threshold,
+                    // the pressure gather, then the queue: the first iteration's
+                    // from the start state (the previous step's counts do not
+                    // find this step's longest chains), the later ones from the
+                    // previous iteration's counts, bubbles whose gathered
+                    // pressure did not change last
+                    launch_queue_keys<F>(s, h->g, h->bp, h->b, p0, h->rp_cur, h->pl_p, loop,
+                                         h->bp.extreme_threshold);
+                    launch_order_partition(s, h->b, h->b.key, h->bp.heavy_threshold,
                                            h->bp.extreme_threshold);
                 }
                 HIPCHK(h, hipEventRecord(h->ev[4], s));
@@ -965,8 +964,8 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
                                                        h->bp.extreme_threshold, h->cus);
                     st.rp_lean_launches += lean ? 1 : 0;
                     launch_bubble_step<F>(s, *h->rpl_h, h->rpl_d, h->rp_cur, h->pl_m, h->pl_n,
-                                          h->pl_p, p0, h->fgb[target], h->sc_d, loop, lean,
-                                          h->cus, h->per_cu);
+                                          h->pl_p, h->fgb[target], h->sc_d, loop, lean, h->cus,
+                                          h->per_cu);
                     if (h->sched_mode == 1) {
                         int rc = schedule_record_call(h);
                         if (rc) return rc;
@@ -1094,6 +1093,7 @@ int run_steps(brp_handle *h, int k, brp_step_stats *stats, StepCtl *ctl = nullpt
             if (mm) status = BRP_ERR_STATE;
         }
         st.rp_max_substeps = h->sc_h->sub_max;
+        st.rp_reused = h->sc_h->reused;
         st.rp_mean_substeps = (loop > 0 && h->nb > 0)
                                   ? (double)h->sc_h->sub_sum / ((double)h->nb * loop)
                                   : 0.0;

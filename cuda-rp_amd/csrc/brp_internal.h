@@ -72,7 +72,7 @@ struct StepScalars {
     int sub_max;                         // max RP sub-steps this step
     unsigned long long sub_sum;          // sum of RP sub-steps this step
     unsigned int done;                   // bubbles written back by the RP kernel this step
-    int pad_;
+    unsigned int reused;                 // of them, kept from the previous iteration
     double focal[2];                     // (x, y) of the largest temperature increment
     unsigned long long resid[kMaxEval];  // max |dp0| bit patterns (>= 0 doubles)
     unsigned int work[kMaxEval];         // bubble work-queue heads, one per Picard iteration
@@ -183,10 +183,12 @@ void launch_pressure_ghosts(hipStream_t s, const GridParams &g, F *p0, const dou
 template <typename F>
 void launch_motion(hipStream_t s, const GridParams &g, const BubbleParams &bp,
                    const BubbleArrays &b, const F *vx, const F *vy, StepScalars *sc);
+// The RP launch of Picard iteration `slot` over the queue order, with the
+// gathered pressures of launch_queue_keys in PL[plp].
 template <typename F>
 void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev, int cur,
-                        int plm, int pln, int plp, const F *p0, int64_t *fg_target,
-                        StepScalars *sc, int slot, bool lean, int cus, int per_cu);
+                        int plm, int pln, int plp, int64_t *fg_target, StepScalars *sc, int slot,
+                        bool lean, int cus, int per_cu);
 // The device's compute units and the persistent RP kernel's workgroups per
 // compute unit (its occupancy limit), queried once per handle.
 void rp_grid_query(int device, int *cus, int *per_cu);
@@ -195,8 +197,12 @@ int bubble_step_blocks(int n_bubbles, bool lean, int cus, int per_cu);
 // longest chain of the last launch is long and the sub-steps per lane of a
 // one-workgroup-per-CU grid are few against it.
 bool bubble_step_lean(int sub_max, double sub_per_launch, int extreme_threshold, int cus);
-// Queue keys of a step's first Picard iteration (predicted sub-step counts).
-void launch_first_keys(hipStream_t s, const BubbleArrays &b, int cur, int ext);
+// Pressure gather of a Picard iteration's bubbles into PL[plp] and its queue
+// keys (predicted or previous sub-step counts; -1: the previous iteration's
+// results stand).
+template <typename F>
+void launch_queue_keys(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                       const BubbleArrays &b, const F *p0, int cur, int plp, int slot, int ext);
 void launch_order_partition(hipStream_t s, const BubbleArrays &b, const int *key, int threshold,
                             int ext);
 void launch_scatter_initial(hipStream_t s, const GridParams &g, const BubbleParams &bp,

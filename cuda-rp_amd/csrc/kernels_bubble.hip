@@ -291,10 +291,12 @@ __device__ inline unsigned long long lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// One Picard iteration for every bubble, fused:
-//   BubbleInterpolationScalarKernel (gather PL_p, kernel.cuh:361-395)
+// One Picard iteration for every bubble, fused (after k_queue_keys gathered
+// PL_p, BubbleInterpolationScalarKernel, kernel.cuh:361-395):
 //   BubbleRadiusKernel              (RP from the step's start state, 471-554)
 //   VoidFractionReverseLookupKernel (scatter R_t into fg_target, 576-619)
+// A bubble whose key is -1 keeps the previous iteration's results (the same
+// inputs) and is only scattered.
 //
 // Persistent kernel with lane-level work refill.  The number of RP sub-steps
 // differs by orders of magnitude between bubbles (about 11 at rest, thousands
@@ -331,7 +333,6 @@ template <typename F, bool FAST>
 __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, int max_substeps,
                                                             const double *rp_shared, int cur,
                                                             int plm, int pln, int plp,
-                                                            const F *__restrict__ p0,
                                                             int64_t *__restrict__ fg_target,
                                                             StepScalars *sc, int slot,
                                                             int solo_cap) {
@@ -378,6 +379,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         E = min(min(b.part[(b.n + 255) / 256], (int)gridDim.x), solo_cap);
     }
     int idx = -1;                  // bubble of this lane, -1 when idle
+    bool reuse = false;            // idx keeps its previous iteration's results
     bool pre = false;
     if (threadIdx.x == 0 && (int)blockIdx.x < E) {
         idx = rpl(kp)->b.order[blockIdx.x];
@@ -395,6 +397,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     int sub_max = 0;
     unsigned long long sub_sum = 0;
     unsigned int done = 0;  // bubbles this lane wrote back
+    unsigned int kept = 0;  // of them, with the previous iteration's results
     bool bad = false, capped_any = false;
     while (true) {
         if (solo && __ballot(idx >= 0) == 0ull) {
@@ -439,12 +442,17 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
             y = b.y[idx];
             ibnx = bin_index(x, g.rdx, 0.5);
             ibny = bin_index(y, g.rdy, 0.5);
-            const double PL = gather_p0(g, bp, p0, x, y, ibnx, ibny);
-            b.PL[plp][idx] = PL;
+            const double PL = b.PL[plp][idx];  // gathered by k_queue_keys
+            reuse = b.key[idx] < 0;            // the previous iteration's results stand
             const double R0 = b.R0[idx];
             k = RPConst{R0, R0 * R0 * R0, b.PG0[idx], b.calpha[idx]};
             if (FAST) fk = rp_fast_const(R0, k.PG0, k.calpha, bp.T0, fsh);
-            if (FAST) {
+            if (reuse) {
+                LF.remain = 0.0;
+                LF.counter = 0;
+                L.remain = 0.0;
+                L.counter = 0;
+            } else if (FAST) {
                 rp_begin_g(LF, b.R[cur][idx], b.Rn[cur][idx], b.d1R[cur][idx], b.PG[cur][idx],
                            b.bdt[cur][idx], b.re[cur][idx], b.PL[plm][idx], b.PL[pln][idx], PL,
                            g.dt, bp.PL0);
@@ -503,30 +511,37 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
             const BubbleParams &bp = P->bp;
             const BubbleArrays &b = P->b;
             if (fin) {
-                RPState out;
-                double R_t, Q_B;
-                if (FAST)
-                    rp_end_g(LF, g.dt, &out, &R_t, &Q_B);
-                else
-                    rp_end(L, g.dt, &out, &R_t, &Q_B);
-                const int cnt = FAST ? LF.counter : L.counter;
-                b.R[nxt][idx] = out.Rp;
-                b.Rn[nxt][idx] = out.Rn;
-                b.d1R[nxt][idx] = out.d1R;
-                b.PG[nxt][idx] = out.PG;
-                b.bdt[nxt][idx] = out.dt;
-                b.re[nxt][idx] = out.re;
-                b.R_t[idx] = R_t;
-                b.Q_B[idx] = Q_B;
-                b.sub[idx] = cnt;
+                double R_t;
+                if (reuse) {  // the previous iteration's results are in place
+                    R_t = b.R_t[idx];
+                    kept++;
+                } else {
+                    RPState out;
+                    double Q_B;
+                    if (FAST)
+                        rp_end_g(LF, g.dt, &out, &R_t, &Q_B);
+                    else
+                        rp_end(L, g.dt, &out, &R_t, &Q_B);
+                    const int cnt = FAST ? LF.counter : L.counter;
+                    b.R[nxt][idx] = out.Rp;
+                    b.Rn[nxt][idx] = out.Rn;
+                    b.d1R[nxt][idx] = out.d1R;
+                    b.PG[nxt][idx] = out.PG;
+                    b.bdt[nxt][idx] = out.dt;
+                    b.re[nxt][idx] = out.re;
+                    b.R_t[idx] = R_t;
+                    b.Q_B[idx] = Q_B;
+                    b.sub[idx] = cnt;
+                    sub_max = max(sub_max, cnt);
+                    sub_sum += (unsigned long long)cnt;
+                    capped_any |= capped && remain > 0.0;
+                }
                 done++;
-                sub_max = max(sub_max, cnt);
-                sub_sum += (unsigned long long)cnt;
-                capped_any |= capped && remain > 0.0;
                 bad |= !isfinite(R_t);
                 scatter_bubble_agg(g, bp, x, y, ibnx, ibny,
                                    fg_temp_of(g, bp, b.n_B[idx], R_t, x), agg, fg_target, &bad);
                 idx = -1;
+                reuse = false;
             }
         }
         if (__builtin_amdgcn_readfirstlane(agg.count) > kAggSlots / 2) agg_flush(agg, lane, fg_target);
@@ -536,9 +551,11 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         sub_max = max(sub_max, __shfl_xor(sub_max, off, 64));
         sub_sum += __shfl_xor(sub_sum, off, 64);
         done += __shfl_xor(done, off, 64);
+        kept += __shfl_xor(kept, off, 64);
     }
     if (lane == 0) {
         if (done) atomicAdd(&sc->done, done);
+        if (kept) atomicAdd(&sc->reused, kept);
         if (sub_sum) atomicAdd(&sc->sub_sum, sub_sum);
         if (sub_max > __hip_atomic_load(&sc->sub_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&sc->sub_max, sub_max);
@@ -551,8 +568,10 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
 // Record: every bubble's RP integration of this Picard iteration again, from
 // the same inputs (the RP start state, the pressure history and the PL_p the
 // RP launch gathered), with the same sub-step function, writing each accepted
-// step size to dts[off[idx] + k]; the count must equal the launch's (a
-// differing count is counted in *mismatch and its extra sizes are dropped).
+// step size to dts[off[idx] + k]; the count must equal the launch's, and the
+// results the launch left (computed, or kept from the previous iteration)
+// must equal these bit for bit (a difference is counted in *mismatch, and a
+// count's extra sizes are dropped).
 __global__ __launch_bounds__(256) void k_rp_record(const RPLaunch *kp, int cur, int plm, int pln,
                                                     int plp, const int64_t *off, double *dts,
                                                     unsigned int *mismatch) {
@@ -574,7 +593,15 @@ __global__ __launch_bounds__(256) void k_rp_record(const RPLaunch *kp, int cur, 
         rp_substep_g(L, fk, fs);
         if (L.counter <= cnt) dts[o + L.counter - 1] = L.dtL;
     }
-    if (L.counter != cnt) atomicAdd(mismatch, 1u);
+    RPState out;
+    double R_t, Q_B;
+    rp_end_g(L, P.g.dt, &out, &R_t, &Q_B);
+    const int nxt = cur ^ 1;
+    const bool same = R_t == b.R_t[idx] && Q_B == b.Q_B[idx] && out.Rp == b.R[nxt][idx] &&
+                      out.Rn == b.Rn[nxt][idx] && out.d1R == b.d1R[nxt][idx] &&
+                      out.PG == b.PG[nxt][idx] && out.dt == b.bdt[nxt][idx] &&
+                      out.re == b.re[nxt][idx];
+    if (L.counter != cnt || !same) atomicAdd(mismatch, 1u);
 }
 
 // Replay: one Picard iteration of every bubble with the recorded step sizes
@@ -814,7 +841,7 @@ int bubble_step_blocks(int n_bubbles, bool lean, int cus, int per_cu) {
 
 template <typename F>
 void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev, int cur,
-                        int plm, int pln, int plp, const F *p0, int64_t *fg_target,
+                        int plm, int pln, int plp, int64_t *fg_target,
                         StepScalars *sc, int slot, bool lean, int cus, int per_cu) {
     if (host.b.n <= 0) return;
     const dim3 grid(bubble_step_blocks(host.b.n, lean, cus, per_cu));
@@ -830,11 +857,11 @@ void launch_bubble_step(hipStream_t s, const RPLaunch &host, const RPLaunch *dev
     const int solo_cap = lean ? (int)grid.x : std::max(1, cus / 8);
     if (host.bp.rp_exact)
         hipLaunchKernelGGL((k_bubble_persistent<F, false>), grid, dim3(256), 0, s, dev, cap,
-                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot,
+                           host.bp.rp_shared, cur, plm, pln, plp, fg_target, sc, slot,
                            solo_cap);
     else
         hipLaunchKernelGGL((k_bubble_persistent<F, true>), grid, dim3(256), 0, s, dev, cap,
-                           host.bp.rp_shared, cur, plm, pln, plp, p0, fg_target, sc, slot,
+                           host.bp.rp_shared, cur, plm, pln, plp, fg_target, sc, slot,
                            solo_cap);
 }
 
@@ -1000,36 +1027,66 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int *sub, const int 
     }
 }
 
-// Queue keys of a step's first Picard iteration.  In a cavitating step the
-// longest RP chains belong to bubbles that are already collapsing fast when
-// the step starts, and they are other bubbles every step, so the previous
-// step's counts do not find them.  A bubble shrinking faster than
-// kCollapseSpeed gets the key 10 (-dR/dt) (R / R0) sqrt(kKeyRadius / R0), an
-// estimate of its sub-steps (on the 4096^2 clustered workload the median
-// count is about ten times (-dR/dt) R / R0 up to 200 m/s; ordered by the key,
-// the 16 longest chains of a step sit within its first 150 queue entries,
-// 94 % of the 64 longest within the first 256, and the 4096 longest within
-// the first 26 000 of 4.2 million); every other bubble keeps its previous
-// count, capped below the extreme class.
+// Queue keys of a Picard iteration, with the pressure gather of its bubbles
+// (BubbleInterpolationScalarKernel, kernel.cuh:361-395) done here, before
+// the queue is ordered; PL_p goes to PL[plp] for k_bubble_persistent.
+//  * First iteration of a step: in a cavitating step the longest RP chains
+//    belong to bubbles that are already collapsing fast when the step
+//    starts, and they are other bubbles every step, so the previous step's
+//    counts do not find them.  A bubble shrinking faster than
+//    kCollapseSpeed gets the key 10 (-dR/dt) (R / R0) sqrt(kKeyRadius / R0),
+//    an estimate of its sub-steps (on the 4096^2 clustered workload the
+//    median count is about ten times (-dR/dt) R / R0 up to 200 m/s; ordered
+//    by the key, the 16 longest chains of a step sit within its first 150
+//    queue entries, 94 % of the 64 longest within the first 256, and the
+//    4096 longest within the first 26 000 of 4.2 million); every other
+//    bubble keeps its previous count, capped below the extreme class.
+//  * Later iterations: the previous iteration's sub-step count, or -1 where
+//    the gathered pressure equals the previous iteration's bit for bit.
+//    That iteration integrated the same inputs (start state, pressure
+//    history, PL_p), so its results stand: k_bubble_persistent only
+//    scatters the bubble, from the light end of the queue.  In mixed
+//    precision most cells' FP32 pressure stops changing once the Picard
+//    residual is small (4096^2 clustered, a step's fifth iteration: 92-97 %
+//    of the bubbles; 2048^2, its fourth: 97 %).
 constexpr double kCollapseSpeed = 25.0;  // m/s
 constexpr double kKeyRadius = 3.0e-6;    // m
 
-__global__ __launch_bounds__(256) void k_first_keys(BubbleArrays b, int cur, int ext) {
+template <typename F>
+__global__ __launch_bounds__(256) void k_queue_keys(GridParams g, BubbleParams bp, BubbleArrays b,
+                                                    const F *__restrict__ p0, int cur, int plp,
+                                                    int slot, int ext) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= b.n) return;
-    const double v = b.d1R[cur][idx];
-    int k = min(b.sub[idx], ext - 1);
-    if (v < -kCollapseSpeed) {
-        const double R0 = b.R0[idx];
-        k = (int)fmin(-10.0 * v * (b.R[cur][idx] / R0) * sqrt(kKeyRadius / R0), 1.0e6);
+    const double x = b.x[idx], y = b.y[idx];
+    const double PL =
+        gather_p0(g, bp, p0, x, y, bin_index(x, g.rdx, 0.5), bin_index(y, g.rdy, 0.5));
+    int k;
+    if (slot == 1) {
+        const double v = b.d1R[cur][idx];
+        k = min(b.sub[idx], ext - 1);
+        if (v < -kCollapseSpeed) {
+            const double R0 = b.R0[idx];
+            k = (int)fmin(-10.0 * v * (b.R[cur][idx] / R0) * sqrt(kKeyRadius / R0), 1.0e6);
+        }
+    } else {
+        k = PL == b.PL[plp][idx] ? -1 : b.sub[idx];
     }
+    b.PL[plp][idx] = PL;
     b.key[idx] = k;
 }
 
-void launch_first_keys(hipStream_t s, const BubbleArrays &b, int cur, int ext) {
+template <typename F>
+void launch_queue_keys(hipStream_t s, const GridParams &g, const BubbleParams &bp,
+                       const BubbleArrays &b, const F *p0, int cur, int plp, int slot, int ext) {
     if (b.n <= 0) return;
-    hipLaunchKernelGGL(k_first_keys, dim3((b.n + 255) / 256), dim3(256), 0, s, b, cur, ext);
+    hipLaunchKernelGGL(k_queue_keys<F>, dim3((b.n + 255) / 256), dim3(256), 0, s, g, bp, b, p0,
+                       cur, plp, slot, ext);
 }
+template void launch_queue_keys<double>(hipStream_t, const GridParams &, const BubbleParams &,
+                                        const BubbleArrays &, const double *, int, int, int, int);
+template void launch_queue_keys<float>(hipStream_t, const GridParams &, const BubbleParams &,
+                                       const BubbleArrays &, const float *, int, int, int, int);
 
 void launch_order_partition(hipStream_t s, const BubbleArrays &b, const int *key, int threshold,
                             int ext) {
@@ -1119,10 +1176,10 @@ template void launch_motion<float>(hipStream_t, const GridParams &, const Bubble
                                    const BubbleArrays &, const float *, const float *,
                                    StepScalars *);
 template void launch_bubble_step<double>(hipStream_t, const RPLaunch &, const RPLaunch *, int, int,
-                                         int, int, const double *, int64_t *, StepScalars *,
-                                         int, bool, int, int);
+                                         int, int, int64_t *, StepScalars *, int, bool, int,
+                                         int);
 template void launch_bubble_step<float>(hipStream_t, const RPLaunch &, const RPLaunch *, int, int,
-                                         int, int, const float *, int64_t *, StepScalars *,
-                                         int, bool, int, int);
+                                         int, int, int64_t *, StepScalars *, int, bool, int,
+                                         int);
 
 }  // namespace brp

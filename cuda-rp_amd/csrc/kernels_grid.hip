@@ -479,6 +479,7 @@ __global__ void k_zero_scalars(StepScalars *sc, int nslots) {
         sc->sub_max = 0;
         sc->sub_sum = 0ull;
         sc->done = 0u;
+        sc->reused = 0u;
     }
 }
 

@@ -121,7 +121,8 @@ typedef struct brp_step_stats {
     double resimax;              /* residual of the last pressure evaluation */
     int32_t n_pic;               /* Picard iterations taken (0 -> RP skipped) */
     int32_t rp_max_substeps;     /* max RP sub-steps of any bubble this step */
-    double rp_mean_substeps;     /* mean RP sub-steps per bubble per call    */
+    double rp_mean_substeps;     /* mean RP sub-steps integrated per bubble
+                                    per call (kept results count 0)          */
     int32_t status;              /* BRP_OK or the error of this step         */
     int32_t rp_lean_launches;    /* RP launches of this step on the lean grid
                                     (one workgroup per compute unit)         */
@@ -130,6 +131,11 @@ typedef struct brp_step_stats {
      * the reference's focal-point tracking (determine_focal_point on Work
      * after calculate_temperature, bubble_CUDA.cu:310-324); 0 otherwise. */
     double focal_x, focal_y;
+    /* RP calls of this step that kept the previous Picard iteration's
+     * results: the bubble's gathered pressure was the same, bit for bit, so
+     * the integration had the same inputs (extension: the reference
+     * integrates every bubble in every iteration, with the same results). */
+    int64_t rp_reused;
 } brp_step_stats;
 
 /* Derived scalars (init_grid_size, mix_set_time_increment, init_bub_params). */

@@ -44,12 +44,13 @@ def test_step_and_grid_rooflines_use_the_byte_models():
     b = _bench()
     n = 4096
     st = [{"n_pic": 5, "rp_mean_substeps": 26.0, "rp_max_substeps": 3717,
-           "rp_lean_launches": 5}] * 4
+           "rp_lean_launches": 5, "rp_reused": 3 << 20}] * 4
     ph = {"velocity": 0.4, "motion": 0.6, "pressure": 0.8, "bubble_rp": 90.0, "other": 1.2}
     sm = b.summarize(st, 0.1, ph, n, 1 << 22, "mixed")
     want = n * n * (116 + 56 * 5) + (1 << 22) * (96 + 288 * 5)  # per step
     assert sm["step_bytes"] == want
     assert sm["device_ms_per_step"] == pytest.approx(93.0 / 4)
+    assert sm["rp_kept_fraction"] == pytest.approx(3.0 / 20.0)  # 3 * 2^20 of 5 * 2^22 calls
     r = b.step_roofline(sm, "mixed")
     assert r["bound"] == "hbm" and r["unit"] == "GB/s"
     assert r["achieved"] == pytest.approx(want / (93.0 / 4 * 1e-3) / 1e9)

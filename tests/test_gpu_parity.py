@@ -247,6 +247,37 @@ def test_many_wavefront_schedule_bit_exact(brp):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("nb", [400, 65536])
+def test_kept_rp_results_equal_a_fresh_integration(brp, nb):
+    """Mixed precision, cavitating: from a step's second Picard iteration on,
+    a bubble whose gathered pressure is the same as in the iteration before,
+    bit for bit (most of them once the FP32 pressure field stops changing),
+    keeps that iteration's RP results instead of integrating again.  In
+    frozen-schedule record mode a separate kernel integrates every bubble of
+    every launch again and compares the sub-step count and all results with
+    what the launch left, bit for bit; a difference fails the step.  The run
+    must keep results in some iterations and show no difference, and its
+    fields, radii and Picard counts must equal those of the same run outside
+    record mode."""
+    cfg, b = cases.bubbly_case(64, nb=nb, seed=7, precision=brp.abi.PREC_MIXED)
+    rec = cases.make(brp.Simulation, cfg, b)
+    rec.schedule_mode(1)
+    s1 = rec.step(120)  # raises on a recording difference (BRP_ERR_STATE)
+    plain = cases.make(brp.Simulation, cfg, b)
+    s2 = plain.step(120)
+    kept = sum(s["rp_reused"] for s in s1)
+    calls = nb * sum(s["n_pic"] for s in s1)
+    print(f"{nb} bubbles: {kept} of {calls} RP calls kept the previous iteration's results",
+          flush=True)
+    assert kept > 0
+    assert [s["n_pic"] for s in s1] == [s["n_pic"] for s in s2]
+    assert [s["rp_reused"] for s in s1] == [s["rp_reused"] for s in s2]
+    for n in ("p0", "fg"):
+        np.testing.assert_array_equal(rec.field(n), plain.field(n))
+    for n in ("R_t", "R_p", "PG", "sub"):
+        np.testing.assert_array_equal(rec.bubbles(n), plain.bubbles(n))
+
+
 def test_wide_band_and_edge_bubbles(brp):
     """smooth_delta_band = 4, bubbles straddling the domain edges and outside it."""
     cfg, _ = cases.bubbly_case(48, nb=10)
