@@ -556,6 +556,7 @@ def main():
         "picard_per_step": sm["picard_per_step"],
         "rp_max_substeps": sm["rp_max_substeps"],
         "rp_lean_launches": sm["rp_lean_launches"],
+        "rp_kept_fraction": sm["rp_kept_fraction"],
         "timed_picard_counts": [s["n_pic"] for s in st_m],
         "phase_ms_per_step": sm["phase_ms_per_step"],
         "roofline": rp_roofline(sm, nb, measured_traffic("k_bubble_persistent_4096_counters.md")),
@@ -574,6 +575,7 @@ def main():
                        "bubble_steps_per_s": sd["bubble_steps_per_s"],
                        "picard_per_step": sd["picard_per_step"],
                        "rp_launch_ms": sd["rp_launch_ms"],
+                       "rp_kept_fraction": sd["rp_kept_fraction"],
                        "phase_ms_per_step": sd["phase_ms_per_step"],
                        "step_roofline": step_roofline(sd, "fp64"),
                        "grid_roofline": grid_roofline(ph_d, n, K, mixed=False)}
