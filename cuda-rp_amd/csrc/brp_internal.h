@@ -19,7 +19,7 @@
 #include <stdint.h>
 
 #include "../../include/bubbles_rp.h"
-#include "rp_vtable.h"
+#include "rp_vfit.h"
 
 namespace brp {
 
@@ -123,8 +123,8 @@ struct RPLaunch {
 
 // The shortened-chain sub-step's constant table: the shared constants
 // (rp_table_values, 15 doubles, padded to 64), then the piecewise polynomials
-// of the thermal-impedance functions U and Y of alpha = s (1 + i)
-// (rp_vtable.h), which the kernels stage in LDS.
+// of the thermal-impedance functions V and Y of alpha = s (1 + i)
+// (rp_vfit.h, built for the simulation's gam), which the kernels stage in LDS.
 constexpr int kRpVOffset = 64;
 constexpr int kRpVFunc = kVIntervals * (kVDegree + 1) * 2;  // doubles per function
 constexpr int kRpVSize = 2 * kRpVFunc;
@@ -133,8 +133,8 @@ constexpr int kRpTable = kRpVOffset + kRpVSize;
 // The constants shared by every bubble in the shortened-chain RP sub-step, in
 // the field order of RPFastShared (rp_device.h): PL0, gam, mu, rho, 1/rho,
 // dt0, 3 gam, 3 (gam - 1), 3 (gam - 1) K0 T0, 2 sig, 4 mu, mu^2,
-// -2 sig rho, 4 pi K0, 16 pi mu; then (from kRpVOffset) kVTable.
-__host__ __device__ inline void rp_table_values(const BubbleParams &bp, double v[kRpTable]) {
+// -2 sig rho, 4 pi K0, 16 pi mu; then (from kRpVOffset) the V and Y tables.
+inline void rp_table_values(const BubbleParams &bp, double v[kRpTable]) {
     v[0] = bp.PL0;
     v[1] = bp.gam;
     v[2] = bp.mu;
@@ -151,8 +151,7 @@ __host__ __device__ inline void rp_table_values(const BubbleParams &bp, double v
     v[13] = 4.0 * bp.pi * bp.K0;
     v[14] = 16.0 * bp.pi * bp.mu;
     for (int k = 15; k < kRpVOffset; k++) v[k] = 0.0;
-    const double *vt = &kVTable[0][0][0][0];
-    for (int k = 0; k < kRpVSize; k++) v[kRpVOffset + k] = vt[k];
+    rp_vtable_build(v[7], v + kRpVOffset);
 }
 
 }  // namespace brp

@@ -727,7 +727,7 @@ constexpr int kJacPerWave = 9;
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_rp_jac(
     BubbleParams bp, BubbleArrays b, double dt_mix, const JacSlot *slots, const unsigned *ord,
     unsigned n_items, const int *bins, unsigned *head, unsigned *mismatch, double *buf) {
-    __shared__ double vtab[kRpVSize];  // the U and Y tables
+    __shared__ double vtab[kRpVSize];  // the V and Y tables
     rp_vtab_stage(bp.rp_shared, vtab);
     __syncthreads();
     const RPFastShared fs = rp_shared_load(bp.rp_shared, vtab);

@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
     const int lane = threadIdx.x & 63;
     const int nxt = cur ^ 1;
     __shared__ WaveAgg agg_all[4];  // 25 KB: one table per wavefront
-    __shared__ double vtab[FAST ? kRpVSize : 1];  // 50 KB: the U and Y tables
+    __shared__ double vtab[FAST ? kRpVSize : 1];  // 50 KB: the V and Y tables
     WaveAgg &agg = agg_all[threadIdx.x >> 6];
     agg_clear(agg, lane);
     // The shared RP constants go through LDS into vector registers, held for
@@ -479,8 +479,10 @@ __global__ __launch_bounds__(256) void k_bubble_persistent(const RPLaunch *kp, i
         if (FAST && solo) {
             // a dedicated wavefront: lane 0 runs its bubble to the sync time
             // without the batch bookkeeping of the shared loop below
+            // (lane 0 alone: its conditions are wave-uniform)
             if (active)
-                while (LF.remain > 0.0 && LF.counter < max_substeps) rp_substep_g(LF, fk, fsh);
+                while (cond<true>(LF.remain > 0.0 && LF.counter < max_substeps))
+                    rp_substep_g<double, false, true, true>(LF, fk, fsh);
             capped = active && LF.counter >= max_substeps;
             remain = active ? LF.remain : 0.0;
             fin = active;

@@ -213,12 +213,12 @@ __device__ inline void rp_end(const RPLane &L, double dt_mix, RPState *out, doub
 
 // ---- constants of the shortened-chain sub-step (rp_fast.h) ---------------
 struct RPFastConst {   // per bubble (rp_fast_const)
-    double iR0, PG0, iPG0, calpha, PG0R03, T0_PG0R03, A0, B0;
+    double iR0, PG0, iPG0, calpha_irho, PG0R03, T0_PG0R03, A0, B0;
 };
 struct RPFastShared {  // shared by all bubbles
     double PL0, gam, mu, rho, irho, dt0, g3, k3, C0, two_sig, four_mu, mu2, m2sig_rho,
         four_pi_K0, sixteen_pi_mu;
-    const double *vtab;  // kVTable staged in LDS by the kernel (upsilon_g, lp_inv_g)
+    const double *vtab;  // the V and Y tables (rp_vfit.h) staged in LDS by the kernel
 };
 
 }  // namespace brp

@@ -5,17 +5,20 @@
 // latency, since one collapsing bubble's thousands of sequential sub-steps
 // bound a whole launch:
 //   * alpha = s (1 + i) always (rp_alpha), so the thermal-impedance terms
-//     are functions of s alone: Upsilon's U = (alpha coth alpha - 1)/alpha^2
-//     and Lp's W = 1/(alpha coth alpha - 1) - 3/alpha^2, of which solvePG
-//     needs only Lp/|Lp|^2 = Y/R, Y = 1/conj(W) (U and Y are finite at
-//     s = 0, where the two terms of W cancel).  For |alpha| <= 0.1 the
-//     reference's own Taylor series; up to s = 20 one degree-9 polynomial
-//     per interval of width 1/8 (rp_vtable.h, staged in LDS; error below
-//     2e-16 of |U| and |Y|); for s > 20 coth alpha = 1 to below an ulp, so
-//     both follow from alpha - 1 in closed form (collapsing bubbles spend
-//     about half their sub-steps there);
-//   * omega_N and 1/omega_N from one rsqrt of eta; sqrt(c_alpha omega / R)
-//     and its reciprocal from one rsqrt;
+//     are functions of s alone: Upsilon = 3 gam / (1 + k3 U) with
+//     U = (alpha coth alpha - 1)/alpha^2, and Lp's W = 1/(alpha coth alpha -
+//     1) - 3/alpha^2, of which solvePG needs only Lp/|Lp|^2 = Y/R,
+//     Y = 1/conj(W) (U and Y are finite at s = 0, where the two terms of W
+//     cancel).  For |alpha| <= 0.1 the reference's own Taylor series; up to
+//     s = 20 one degree-9 polynomial per interval of width 1/8 of
+//     V = conj(1 / (1 + k3 U)) and of Y (rp_vfit.h, built for the
+//     simulation's gam and staged in LDS; error below 2e-16 of |V| and |Y|),
+//     so that no division follows the table; for s > 20 coth alpha = 1 to
+//     below an ulp, so both follow from alpha - 1 in closed form (collapsing
+//     bubbles spend about half their sub-steps there);
+//   * omega_N and 1/omega_N from one rsqrt of eta; c_alpha omega / R, its
+//     square root and reciprocal square root from the same rsqrt and one
+//     more;
 //   * reciprocals by v_rcp_f64 and two Newton steps (drcp), reciprocal
 //     square roots by v_rsq_f64 and one second-order step (drsqrt); every
 //     fused multiply-add is written out (the file is compiled with
@@ -90,11 +93,12 @@ __device__ inline dd rsqrt_(dd x) {
     return dd(r, -0.5 * r * r * r * x.d);
 }
 // Per-bubble constants of the sub-step.  A0 and B0 are the Omega iteration's
-// (rp_substep_g): 3 gam rho PG0 R0^3 and -3 gam PG0 R0^3 / 4.
+// (rp_substep_g): 3 gam rho PG0 R0^3 and -3 gam PG0 R0^3 / 4; calpha_irho is
+// c_alpha / rho.
 __device__ inline RPFastConst rp_fast_const(double R0, double PG0, double calpha, double T0,
                                             const RPFastShared &s) {
     const double PG0R03 = PG0 * (R0 * R0 * R0);
-    return RPFastConst{1.0 / R0,      PG0,    1.0 / PG0, calpha, PG0R03, T0 / PG0R03,
+    return RPFastConst{1.0 / R0,      PG0,    1.0 / PG0, calpha * s.irho, PG0R03, T0 / PG0R03,
                        (s.g3 * s.rho) * PG0R03, (-0.25 * s.g3) * PG0R03};
 }
 
@@ -103,7 +107,7 @@ typedef __attribute__((address_space(4))) const double cdouble;
 
 // The shared constants from their device table (rp_table_values), read
 // where the sub-step uses them rather than held in scalar registers across
-// the sub-step loop; vtab is the kernel's LDS copy of the U and Y tables.
+// the sub-step loop; vtab is the kernel's LDS copy of the V and Y tables.
 __device__ inline RPFastShared rp_shared_load(const double *tab, const double *vtab) {
     cdouble *t = (cdouble *)tab;
     asm volatile("" : "+s"(t));
@@ -117,7 +121,7 @@ typedef __attribute__((address_space(3))) const double ldouble;  // LDS
 typedef const double ldouble;  // the host pass only parses device code
 #endif
 
-// Copies the U and Y tables (kRpVSize doubles from the device table) into the
+// Copies the V and Y tables (kRpVSize doubles from the device table) into the
 // workgroup's LDS array; the caller synchronises the workgroup afterwards.
 __device__ inline void rp_vtab_stage(const double *tab, double *lds) {
     for (int k = threadIdx.x; k < kRpVSize; k += blockDim.x) lds[k] = tab[kRpVOffset + k];
@@ -127,7 +131,7 @@ struct cx_d {
     double re, im;
 };
 
-// U or Y (ftab = that function's table in LDS) for 0 <= s <= 20: interval
+// V or Y (ftab = that function's table in LDS) for 0 <= s <= 20: interval
 // k = floor(8 s), t = 16 s - (2k + 1), Horner in t for the real and
 // imaginary parts (two independent chains); with dpr/dpi also d/ds.
 __device__ inline cx_d v_poly(double s, const double *ftab, double *dpr = nullptr,
@@ -197,29 +201,47 @@ __device__ inline bool small_alpha(T s) {
     return !(val(s) > 0.070710678118654752);
 }
 
+// A branch condition of the sub-step.  UNI: the caller runs the sub-step on
+// one lane alone (a dedicated wavefront), so the condition of the active
+// lanes' ballot is that lane's and the branch is taken on the scalar unit,
+// without saving and restoring the execution mask around each arm.
+template <bool UNI>
+__device__ inline bool cond(bool c) {
+    if constexpr (UNI)
+        return __builtin_amdgcn_ballot_w64(c) != 0ull;
+    else
+        return c;
+}
+
 // Upsilon(alpha) = 3 gam / (1 + k3 U) (bubble_CUDA_kernel.cuh:296-311) from
-// s and is = 1/s, U = (alpha coth alpha - 1) / alpha^2: the table up to
-// s = 20; beyond it coth alpha = 1 to below an ulp, so
-// U = (alpha - 1) / alpha^2 = (is / 2) (1 + i (is - 1)).  Returned as
-// conj(Upsilon) / (3 gam) = conj(1 / (1 + k3 U)): the caller's constants carry
-// the factor 3 gam and the sign of the imaginary part.
-template <typename T>
+// s and is = 1/s, U = (alpha coth alpha - 1) / alpha^2, returned as
+// V = conj(Upsilon) / (3 gam) = conj(1 / (1 + k3 U)): the caller's constants
+// carry the factor 3 gam and the sign of the imaginary part.  V's table up to
+// s = 20 (k3 is the simulation's; the table was built for it); beyond it
+// coth alpha = 1 to below an ulp, so U = (alpha - 1) / alpha^2 =
+// (is / 2) (1 + i (is - 1)).
+template <typename T, bool UNI = false>
 __device__ inline cx<T> upsilon_g(T s, T is, double k3, const double *vtab) {
-    cx<T> u;
-    if (val(s) > 20.0) {
-        const T h = 0.5 * is;
-        u = cx<T>{h, h * (is - T(1.0))};
-    } else if (small_alpha(s)) {  // 1/3 + a^2 (-1/45 + a^2 (2/945 - a^2/4725)), a^2 = i q
-        const T q = 2.0 * (s * s);
-        const T ur = T(2.0 / 945.0), ui = -(q * (1.0 / 4725.0));
-        const T nr = fma_(-q, ui, T(-1.0 / 45.0)), ni = q * ur;
-        u = cx<T>{fma_(-q, ni, T(1.0 / 3.0)), q * nr};
+    cx<T> v;
+    const bool far = cond<UNI>(val(s) > 20.0), small = cond<UNI>(small_alpha(s));
+    if (!far && !small) {  // the table
+        v = v_table(s, vtab);
     } else {
-        u = v_table(s, vtab);
+        cx<T> u;
+        if (far) {
+            const T h = 0.5 * is;
+            u = cx<T>{h, h * (is - T(1.0))};
+        } else {  // 1/3 + a^2 (-1/45 + a^2 (2/945 - a^2/4725)), a^2 = i q
+            const T q = 2.0 * (s * s);
+            const T ur = T(2.0 / 945.0), ui = -(q * (1.0 / 4725.0));
+            const T nr = fma_(-q, ui, T(-1.0 / 45.0)), ni = q * ur;
+            u = cx<T>{fma_(-q, ni, T(1.0 / 3.0)), q * nr};
+        }
+        const T dr = fma_(T(k3), u.re, T(1.0)), di = k3 * u.im;
+        const T f = rcp_(fma_(dr, dr, di * di));
+        v = cx<T>{dr * f, di * f};
     }
-    const T dr = fma_(T(k3), u.re, T(1.0)), di = k3 * u.im;
-    const T f = rcp_(fma_(dr, dr, di * di));
-    return cx<T>{dr * f, di * f};
+    return v;
 }
 
 // solveLp (bubble_CUDA_kernel.cuh:314-320) as solvePG uses it
@@ -229,15 +251,16 @@ __device__ inline cx<T> upsilon_g(T s, T is, double k3, const double *vtab) {
 // Y = conj(N) / conj(D) with N = alpha^2 (alpha - 1), D = alpha^2 - 3 alpha + 3,
 // both divided by s^2: conj(N)/s^2 = -2s - 2i(s - 1),
 // conj(D)/s^2 = 3 is (is - 1) - i (2 - 3 is).
-template <typename T>
+template <typename T, bool UNI = false>
 __device__ inline cx<T> lp_inv_g(T s, T is, T iR, const double *vtab) {
     cx<T> y;
-    if (val(s) > 20.0) {
+    const bool far = cond<UNI>(val(s) > 20.0), small = cond<UNI>(small_alpha(s));
+    if (far) {
         const T a = -2.0 * s, b = -2.0 * (s - T(1.0));
         const T c = (3.0 * is) * (is - T(1.0)), d = fma_(T(3.0), is, T(-2.0));
         const T id = rcp_(fma_(c, c, d * d));
         y = cx<T>{fma_(a, c, b * d) * id, fma_(b, c, -(a * d)) * id};
-    } else if (small_alpha(s)) {  // W = 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875))
+    } else if (small) {  // W = 1/5 + a^2 (-1/175 + a^2 (2/7875 - 37 a^2 / 3031875))
         const T q = 2.0 * (s * s);
         const T ur = T(2.0 / 7875.0), ui = -(q * (37.0 / 3031875.0));
         const T nr = fma_(-q, ui, T(-1.0 / 175.0)), ni = q * ur;
@@ -250,12 +273,14 @@ __device__ inline cx<T> lp_inv_g(T s, T is, T iR, const double *vtab) {
     return cx<T>{y.re * iR, y.im * iR};
 }
 
-// omega = sqrt(max(eta, floor)) / (rho R) and 1 / omega from one rsqrt.
+// omega = sqrt(max(eta, floor)) / (rho R) enters only through
+// x = c_alpha omega / R, and 1 / omega; both from one rsqrt, with
+// xscale = c_alpha / (rho R^2) and iscale = rho R.
 template <typename T>
-__device__ inline void omega_g(T eta, T scale, T iscale, T *om, T *iom) {
+__device__ inline void omega_g(T eta, T xscale, T iscale, T *x, T *iom) {
     const T e = val(eta) >= 1.0e-6 * kDblEps ? eta : T(1.0e-6 * kDblEps);
     const T r = rsqrt_(e);
-    *om = (e * r) * scale;
+    *x = (e * r) * xscale;
     *iom = r * iscale;
 }
 
@@ -293,12 +318,13 @@ __device__ inline void rp_begin_g(RPLaneF<T> &L, T Rp, T Rn, T d1R, T PG, double
 // denominator exceeds DBL_EPSILON).  The two limits share one reciprocal
 // square root: min(0.01 R / |dR/dt|, 0.1 R / sqrt|R d2R/dt2|) =
 // R / sqrt(max(1e4 (dR/dt)^2, 100 |R d2R/dt2|)).
+template <bool UNI = false>
 __device__ inline double rp_dtp(double dtm, double Rn, double d1R, double Rnd2R,
                                 const RPFastShared &s) {
     double dtp = fmin(1.01 * dtm, s.dt0);
     const double ad = fabs(d1R), aa = fabs(Rnd2R);
     const double m = fmax(ad > kDblEps ? 1.0e4 * (ad * ad) : 0.0, aa > kDblEps ? 100.0 * aa : 0.0);
-    if (m > 0.0) dtp = fmin(dtp, Rn * drsqrt(m));
+    if (__builtin_expect(cond<UNI>(m > 0.0), 1)) dtp = fmin(dtp, Rn * drsqrt(m));
     return dtp;
 }
 
@@ -309,7 +335,7 @@ __device__ inline double rp_dtp(double dtm, double Rn, double d1R, double Rnd2R,
 // Upsilon.re coef1 coef2 = (Upsilon.re / 3 gam) A and
 // Upsilon.im coef1 PGn / 4 = (-Upsilon.im / 3 gam) B with A = A0 / Rn^3,
 // B = B0 / Rn^3 (rp_fast_const), so both depend on Rn alone.
-template <typename T>
+template <typename T, bool UNI = false>
 __device__ inline void omega_lp_g(T Rn, T iRn, const RPFastConst &k, const RPFastShared &s,
                                   cx<T> *Ln, T *iom_out) {
     const T iRn2 = iRn * iRn;
@@ -317,23 +343,21 @@ __device__ inline void omega_lp_g(T Rn, T iRn, const RPFastConst &k, const RPFas
     const T A = k.A0 * iRn3, B = k.B0 * iRn3;
     const T coef3 = 4.0 * iRn2;
     const T value1 = s.m2sig_rho * iRn;
-    const T scale = iRn * s.irho;
+    const T xscale = k.calpha_irho * iRn2;  // c_alpha / (rho Rn^2)
     const T iscale = Rn * s.rho;
     T eta = A + fma_(T(-s.mu2), coef3, value1);
-    T om, iom;
-    omega_g(eta, scale, iscale, &om, &iom);
-    const T ca = k.calpha * iRn;
+    T x, iom;  // x = c_alpha omega / Rn = s^2
+    omega_g(eta, xscale, iscale, &x, &iom);
+#pragma unroll
     for (int it = 0; it < 3; it++) {
-        const T x = ca * om;
         const T isa = rsqrt_(x);
-        const cx<T> ups = upsilon_g(x * isa, isa, s.k3, s.vtab);
+        const cx<T> ups = upsilon_g<T, UNI>(x * isa, isa, s.k3, s.vtab);
         const T mu_eff = fma_(ups.im * B, iom, T(s.mu));  // mu + ui PG / (4 omega)
         eta = fma_(ups.re, A, fma_(-(mu_eff * mu_eff), coef3, value1));
-        omega_g(eta, scale, iscale, &om, &iom);
+        omega_g(eta, xscale, iscale, &x, &iom);
     }
-    const T x = ca * om;
     const T isa = rsqrt_(x);
-    *Ln = lp_inv_g(x * isa, isa, iRn, s.vtab);
+    *Ln = lp_inv_g<T, UNI>(x * isa, isa, iRn, s.vtab);
     *iom_out = iom;
 }
 
@@ -341,7 +365,8 @@ __device__ inline void omega_lp_g(T Rn, T iRn, const RPFastConst &k, const RPFas
 // FORCED: the step size is forced_dtp instead of the step-size control's
 // (the replay of a recorded schedule; the rest of the sub-step is unchanged).
 // HEAT: accumulate the heat and viscous dissipation of Q_B (T = double).
-template <typename T, bool FORCED = false, bool HEAT = true>
+// UNI: one lane runs the sub-step alone (branches on the scalar unit, cond).
+template <typename T, bool FORCED = false, bool HEAT = true, bool UNI = false>
 __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const RPFastShared &s,
                                     double forced_dtp = 0.0) {
     // solveRayleighPlesset (bubble_CUDA_kernel.cuh:227-260)
@@ -355,7 +380,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     const T sv = iRn * fma_(T(s.four_mu), d1R, T(s.two_sig));  // (2 sig + 4 mu dR) / R
     const T Rnd2R = fma_(-1.5 * d1R, d1R, (PGn - PL - sv) * s.irho);
     const double dtm = L.dtL;
-    double dtp = rp_dtp(dtm, val(Rn), val(d1R), val(Rnd2R), s);
+    double dtp = rp_dtp<UNI>(dtm, val(Rn), val(d1R), val(Rnd2R), s);
     if constexpr (FORCED) dtp = forced_dtp;
     const double idtm = drcp(dtm);
     const double r = dtp * idtm;
@@ -365,7 +390,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     const double iden = drcp(dtm * dtp * (dtm + dtp));
     d1R = fma_(fma_(T(dm2), Rp, fma_(T(dp2 - dm2), Rn, -dp2 * Rm)), T(iden), acc);
     const double dtL = dtp;
-    if (dtp >= L.remain) {  // last sub-step: R_t at the sync time
+    if (__builtin_expect(cond<UNI>(dtp >= L.remain), 0)) {  // last sub-step: R_t at the sync time
         const double d = L.remain;
         const double r2 = d * idtm;
         L.Rt = fma_((d * Rnd2R) * iRn, T(0.5 * (d + dtm)), fma_(T(1.0 + r2), Rn, -r2 * Rm));
@@ -381,7 +406,7 @@ __device__ inline void rp_substep_g(RPLaneF<T> &L, const RPFastConst &k, const R
     L.d1R = d1R;
     cx<T> Ln;  // Lp / |Lp|^2
     T iom;
-    omega_lp_g(Rn, iRn, k, s, &Ln, &iom);
+    omega_lp_g<T, UNI>(Rn, iRn, k, s, &Ln, &iom);
     // solvePG (bubble_CUDA_kernel.cuh:322-340)
     const T Rbar = 0.5 * (Rp + Rn);
     const T iRbar = rcp_(Rbar);
